@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Benchmark: GP iterations/s of the electrostatic placement loop on one MI355X.
+
+A step is one Placement::step (placement.rs:180-228) over the 1M-cell workload
+(1,000,000 movable cells, 1,050,000 nets, degree cap 256, 300 macros,
+utilisation 0.8, 1024x1024 bins), synthetic netlist from the seeded generator.
+Inputs are resident in HBM before timing; W warmup iterations run on a scratch
+context state which is then reset, and K iterations are timed from iteration 0
+(the reference's own loop starts there, main.rs:78).
+
+Prints one JSON line (rank 0) with the metric, a roofline object for the
+dominant kernel (its average launch time measured with HIP events on the
+context's stream) and the CPU baseline (the oracle's f32 OpenMP path on a
+bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "reda-erplacer_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "GP iters/s @1M cells & placements/min + p50/p99 latency @ HPWL within 0.5%"
+
+
+def kernel_bytes(name, d, grid, cells_pins):
+    """Algorithmic HBM bytes of one launch of `name` (each logical array read or
+    written once per launch)."""
+    import numpy as np
+    C, E, P = d.num_movable, d.num_nets, d.num_pin_total
+    deg = np.diff(d.net_start)
+    B = grid * grid
+    n2 = int(d.n_2nets)
+    multi_small = deg[n2:][deg[n2:] <= 32]
+    multi_large = deg[n2:][deg[n2:] > 32]
+    table = {
+        # slot node (4) + pin offset (8) + instance position gather (8) + pin grad write (8)
+        "wa_2pin": n2 * 2 * 28,
+        "wa_small": int(multi_small.sum()) * 28 + multi_small.size * 4,
+        "wa_large": int(multi_large.sum()) * 28 + multi_large.size * 4,
+        # cell position, clamped size, offset (8 each) + ratio (4); int64 map written once
+        "scatter": C * 28 + B * 8,
+        # read q_mov, q_fix (16), clear q_mov (8), write density (4) and row transform (4)
+        "pass1_row_dct": B * 32,
+        # read row transform (4), write three column-inverse maps (12)
+        "pass2_col_spectral": B * 16,
+        # read three maps (12) + density (4), write field x/y and potential (12)
+        "pass3_row_final": B * 28,
+        # per cell: CSR start (4), pin slots (4+8 per pin), position/offset/size/ratio (28),
+        # pins/area (8), previous position and gradient (16), gradient write (8),
+        # field maps read once (8 per bin)
+        "cell_fused": C * (4 + 28 + 8 + 16 + 8) + cells_pins * 12 + B * 8,
+        # position, previous position, gradient, clamp bound read; three writes
+        "update": C * 56,
+        "scalar": 0,
+    }
+    return table.get(name, 0)
+
+
+def cpu_baseline(d, grid, td, iters, threads):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    P = O.Placer(d, grid, target_density=td, sfx="_f32", nthreads=threads)
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        P.step()
+    dt = time.perf_counter() - t0
+    return iters / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=800)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="large")
+    ap.add_argument("--spectral", default="fft", choices=["fft", "mfma"])
+    ap.add_argument("--timing-steps", type=int, default=100,
+                    help="iterations re-run with per-kernel HIP events for the roofline")
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="only the timed loop (for rocprofv3 runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(jobs shard across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from erplacer import lib as L
+    from erplacer import workloads as W
+    from erplacer.placement import Placement
+
+    t0 = time.perf_counter()
+    d, grid, iters_cfg, td = W.make(args.workload)
+    gen_s = time.perf_counter() - t0
+    print(f"[bench] workload {args.workload}: {d.summary()} grid {grid} (generated in {gen_s:.1f}s)",
+          file=sys.stderr, flush=True)
+    mode = L.SPECTRAL_FFT if args.spectral == "fft" else L.SPECTRAL_MFMA_BF16X3
+    P = Placement(d, grid, target_density=td, spectral_mode=mode)
+    P.set_store_gradients(False)
+    if args.warmup:
+        P.run(args.warmup)
+    P.reset()
+    P.info()  # synchronise
+    t0 = time.perf_counter()
+    info = P.run(args.steps)   # returns after a stream synchronize
+    elapsed = time.perf_counter() - t0
+    ips = args.steps / elapsed
+    print(f"[bench] timed {args.steps} iterations in {elapsed:.3f}s -> {ips:.1f} it/s; "
+          f"hpwl {info['hpwl']:.6e} overflow {info['overflow']:.5f}", file=sys.stderr, flush=True)
+    if args.profile:
+        return
+    hpwl_dbu = P.final_hpwl_dbu()
+
+    # per-kernel device time with HIP events over a second timed region
+    P.reset()
+    P.set_timing(True)
+    P.run(min(args.timing_steps, args.steps))
+    P.set_timing(False)
+    times = P.timing()
+    P.set_timing(False)
+    pins_cells = int(sum(int(d.node_pin_start[c + 1] - d.node_pin_start[c]) for c in [0]) * 0)
+    pins_cells = int(d.node_pin_start[d.num_movable])
+    per_iter = {}
+    for k, ms in times.items():
+        per_iter[k] = ms
+    dom = max((k for k in per_iter if k != "scalar"), key=lambda k: per_iter[k])
+    dom_ms = per_iter[dom]
+    nbytes = kernel_bytes(dom, d, grid, pins_cells)
+    achieved = nbytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": nbytes, "avg_launch_ms": round(dom_ms, 5)}
+    kernels = {k: {"avg_ms": round(v, 5),
+                   "gbs": round(kernel_bytes(k, d, grid, pins_cells) / (v * 1e-3) / 1e9, 1) if v > 0 else None}
+               for k, v in per_iter.items()}
+    print(f"[bench] per-kernel avg ms: {kernels}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cps, cdt = cpu_baseline(d, grid, td, args.cpu_iters, threads)
+        cpu = {"value": round(cps, 4), "unit": "GP iterations/s", "cores": threads, "kind": "port",
+               "sample": f"first {args.cpu_iters} iterations of the same workload, f32, "
+                         f"OpenMP restatement of the Rayon path ({cdt:.1f}s)"}
+        print(f"[bench] cpu baseline {cps:.4f} it/s on {threads} threads", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC, "value": round(ips, 2), "unit": "GP iterations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded generator, reda-db contract)",
+        "config": {"workload": f"{args.workload}: {d.num_movable} cells, {d.num_nets} nets, "
+                               f"{d.num_pin_total} pins, {grid}x{grid} bins",
+                   "iterations_of_workload": iters_cfg, "spectral": args.spectral,
+                   "parallelism": "single GPU"},
+        "final": {"hpwl": info["hpwl"], "overflow": info["overflow"], "hpwl_dbu": hpwl_dbu,
+                  "iteration": info["iteration"]},
+        "roofline": roof, "kernels": kernels, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

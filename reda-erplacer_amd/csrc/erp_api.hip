@@ -1,0 +1,981 @@
+// C ABI of liberp_gpu.so: the placement context (Placement::new and
+// Placement::step of placement.rs:159-228 on the device), the per-phase entry
+// points, the hipGraph-replayed iteration loop and the stand-alone leaf kernels.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+
+#include "erp_internal.h"
+#include "erp_kernels.h"
+#include "erp_timer.h"
+
+namespace erp {
+
+static thread_local std::string g_error;
+void set_error(const std::string &msg) { g_error = msg; }
+int fail(int code, const std::string &msg) {
+    set_error(msg);
+    return code;
+}
+
+constexpr int kSmallDegree = 32;     // nets up to this degree: one per thread
+constexpr int kGraphIters = 8;       // iterations per captured graph
+
+double fx_scale(double mass_bound) {
+    if (!(mass_bound > 0)) return 1.0;
+    int e;
+    std::frexp(4611686018427387904.0 / mass_bound, &e);
+    return std::ldexp(1.0, e - 1);
+}
+
+}  // namespace erp
+
+using namespace erp;
+
+struct erp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t N = 0, C = 0, F = 0, P = 0, E = 0, n2 = 0, Pm = 0;
+    int m = 0, n = 0;
+    int64_t B = 0;
+    float bw = 0, bh = 0, die_w = 0, die_h = 0, t = 0;
+    double movable_area = 0, fx_scale = 1, coef = 0, gamma0 = 0, dw0 = 0;
+    int64_t coef_q = 0;
+    int spectral_mode = ERP_SPECTRAL_FFT;
+    int64_t iter_host = 0;
+    bool debug_parts = true;   // keep per-cell WA / density gradients in the fused loop
+    bool keep_q = false;       // keep the movable accumulators after pass 1
+    bool q_dirty = false;
+    // device buffers
+    DevBuf d_x, d_y, d_w, d_h, d_npins_all;
+    DevBuf pos, prev_pos, prev_g, g, gwl, gd;
+    DevBuf csize, coff, ratio, area, npins, hi;
+    DevBuf slot_node, slot_off, net_start, slot_grad, cstart, cslots;
+    DevBuf q_mov, q_fix, rho, rowt, p1, p2, p3, fx, fy, pot, auv;
+    DevBuf hpwl_part, ovf_part, energy_part, part_a, part_b;
+    DevBuf state, gamma_override, hpwl_dbu;
+    SpecTables tm, tn;
+    MfmaPlan *mfma = nullptr;
+    WaNets wa{};
+    CellBufs cb{};
+    PotentialBufs pb{};
+    ScalarArgs sa{};
+    int n_hpwl_part = 0, n_pot_part = 0, n_cell_part = 0;
+    hipGraphExec_t graph_many = nullptr, graph_one = nullptr;
+    KernelTimer *timer = nullptr;
+    bool timing = false;
+};
+
+namespace {
+
+int check_grid(int v) {
+    return v >= kMinBins && v <= kMaxBins && is_pow2(v);
+}
+
+struct DeviceGuard {
+    int rc = ERP_OK;
+    explicit DeviceGuard(int dev) {
+        hipError_t e = hipSetDevice(dev);
+        if (e != hipSuccess) rc = fail(ERP_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    }
+};
+
+template <class T> int upload(DevBuf &b, const T *src, size_t count, hipStream_t s) {
+    int rc = b.alloc(sizeof(T) * count);
+    if (rc) return rc;
+    if (count) ERP_HIP_TRY(hipMemcpyAsync(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice, s));
+    return ERP_OK;
+}
+
+int validate(const erp_design *d, const erp_config *c) {
+    if (!d || !c) return fail(ERP_ERR_INVALID, "null design or config");
+    if (!check_grid(c->num_bins_x) || !check_grid(c->num_bins_y))
+        return fail(ERP_ERR_UNSUPPORTED, "bin grid must be a power of two in [64, 2048] per axis");
+    if (d->num_nodes <= 0 || d->num_movable < 0 || d->num_movable > d->num_nodes)
+        return fail(ERP_ERR_INVALID, "bad num_nodes / num_movable");
+    if (d->num_nodes > INT32_MAX || d->num_pins_total > INT32_MAX)
+        return fail(ERP_ERR_UNSUPPORTED, "more than 2^31 instances or pins");
+    if (!d->x || !d->y || !d->w || !d->h || !d->num_pins)
+        return fail(ERP_ERR_INVALID, "missing instance arrays");
+    if (d->num_pins_total > 0 && (!d->pin_offset_x || !d->pin_offset_y || !d->pin_node))
+        return fail(ERP_ERR_INVALID, "missing pin arrays");
+    if (d->num_nets < 0 || d->n_2nets < 0 || d->n_2nets > d->num_nets)
+        return fail(ERP_ERR_INVALID, "bad num_nets / n_2nets");
+    if (d->num_nets > 0 && (!d->net_start || !d->net_pins))
+        return fail(ERP_ERR_INVALID, "missing net arrays");
+    if (!d->node_pin_start || !d->node_pins) return fail(ERP_ERR_INVALID, "missing macro_2_pins");
+    if (!(d->die_w > 0) || !(d->die_h > 0)) return fail(ERP_ERR_INVALID, "empty die");
+    if (!(d->movable_area >= 0)) return fail(ERP_ERR_INVALID, "bad movable_area");
+    return ERP_OK;
+}
+
+int timer_kernel_names(erp_ctx *ctx, double *ms, int32_t cap, int32_t *count, const char **names) {
+    if (!ctx->timer) { *count = 0; return ERP_OK; }
+    return ctx->timer->report(ms, cap, count, names);
+}
+
+// ---------------------------------------------------------------- phases
+int phase_hpwl(erp_ctx *ctx, const double *gamma_dev) {
+    KernelTimer *t = ctx->timing ? ctx->timer : nullptr;
+    int rc = launch_wa(ctx->wa, ctx->pos.as<float2>(), gamma_dev, ctx->stream, t);
+    if (rc) return rc;
+    if ((rc = launch_pin2cell(ctx->cb, ctx->stream))) return rc;
+    return launch_scalar(ctx->sa, SC_HPWL, ctx->stream);
+}
+
+int run_potential(erp_ctx *ctx) {
+    KernelTimer *t = ctx->timing ? ctx->timer : nullptr;
+    int rc;
+    if (ctx->q_dirty) {
+        if ((rc = launch_zero_i64(ctx->q_mov.as<int64_t>(), ctx->B, ctx->stream))) return rc;
+        ctx->q_dirty = false;
+    }
+    if (t) t->begin("scatter");
+    rc = launch_scatter(ctx->pos.as<float2>(), ctx->csize.as<float2>(), ctx->coff.as<float2>(),
+                        ctx->ratio.as<float>(), ctx->C, ctx->m, ctx->n, ctx->bw, ctx->bh,
+                        ctx->fx_scale, ctx->q_mov.as<int64_t>(), ctx->stream);
+    if (t) t->end();
+    if (rc) return rc;
+    PotentialBufs b = ctx->pb;
+    b.keep_q = ctx->keep_q;
+    if (ctx->spectral_mode == ERP_SPECTRAL_MFMA_BF16X3)
+        rc = launch_potential_mfma(b, ctx->mfma, ctx->stream, t);
+    else
+        rc = launch_potential_fft(b, ctx->tm, ctx->tn, ctx->stream, t);
+    if (ctx->keep_q) ctx->q_dirty = true;
+    return rc;
+}
+
+int phase_potential(erp_ctx *ctx) {
+    int rc = run_potential(ctx);
+    if (rc) return rc;
+    if ((rc = launch_efield(ctx->cb, ctx->stream))) return rc;
+    return launch_scalar(ctx->sa, SC_POT, ctx->stream);
+}
+
+int phase_nesterov(erp_ctx *ctx) {
+    DevState *st = ctx->state.as<DevState>();
+    int rc;
+    if (ctx->iter_host == 0) {
+        if ((rc = launch_l1(ctx->cb, ctx->stream))) return rc;
+        if ((rc = launch_scalar(ctx->sa, SC_L1, ctx->stream))) return rc;
+    }
+    if ((rc = launch_combine(ctx->cb, st, ctx->stream))) return rc;
+    if ((rc = launch_scalar(ctx->sa, SC_NEST, ctx->stream))) return rc;
+    return launch_update(ctx->cb, st, ctx->stream);
+}
+
+int phase_scheduler(erp_ctx *ctx) {
+    int rc = launch_scalar(ctx->sa, SC_SCHED, ctx->stream);
+    if (rc == ERP_OK) ctx->iter_host += 1;
+    return rc;
+}
+
+// One iteration (iteration >= 1) as a fused kernel sequence: what the graphs hold.
+int launch_fused_iteration(erp_ctx *ctx) {
+    KernelTimer *t = ctx->timing ? ctx->timer : nullptr;
+    DevState *st = ctx->state.as<DevState>();
+    int rc = launch_wa(ctx->wa, ctx->pos.as<float2>(), &st->gamma, ctx->stream, t);
+    if (rc) return rc;
+    if ((rc = run_potential(ctx))) return rc;
+    if (t) t->begin("cell_fused");
+    rc = launch_cell_fused(ctx->cb, st, ctx->debug_parts, ctx->stream);
+    if (t) t->end();
+    if (rc) return rc;
+    if (t) t->begin("scalar");
+    rc = launch_scalar(ctx->sa, SC_HPWL | SC_POT | SC_NEST | SC_SCHED, ctx->stream);
+    if (t) t->end();
+    if (rc) return rc;
+    if (t) t->begin("update");
+    rc = launch_update(ctx->cb, st, ctx->stream);
+    if (t) t->end();
+    return rc;
+}
+
+void drop_graphs(erp_ctx *ctx) {
+    if (ctx->graph_many) (void)hipGraphExecDestroy(ctx->graph_many);
+    if (ctx->graph_one) (void)hipGraphExecDestroy(ctx->graph_one);
+    ctx->graph_many = ctx->graph_one = nullptr;
+}
+
+int capture(erp_ctx *ctx, int iters, hipGraphExec_t *out) {
+    ERP_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = ERP_OK;
+    for (int k = 0; k < iters && rc == ERP_OK; ++k) rc = launch_fused_iteration(ctx);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(ERP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(ERP_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return ERP_OK;
+}
+
+int read_info(erp_ctx *ctx, erp_step_info *info) {
+    if (!info) return ERP_OK;
+    DevState s;
+    ERP_HIP_TRY(hipMemcpyAsync(&s, ctx->state.p, sizeof(DevState), hipMemcpyDeviceToHost, ctx->stream));
+    ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    info->hpwl = s.hpwl;
+    info->overflow = s.overflow;
+    info->energy = s.energy;
+    info->density_weight = s.density_weight;
+    info->gamma = s.gamma;
+    info->alpha = s.alpha;
+    info->beta = s.beta;
+    info->iteration = s.iteration;
+    info->should_stop = s.should_stop;
+    info->nonfinite = s.nonfinite;
+    return ERP_OK;
+}
+
+int reset_ctx(erp_ctx *ctx) {
+    int rc = launch_init_positions(ctx->pos.as<float2>(), ctx->d_w.as<float>(), ctx->d_h.as<float>(),
+                                   ctx->d_x.as<float>(), ctx->d_y.as<float>(), ctx->N, ctx->C,
+                                   ctx->die_w, ctx->die_h, ctx->prev_pos.as<float2>(),
+                                   ctx->prev_g.as<float2>(), ctx->stream);
+    if (rc) return rc;
+    if ((rc = launch_reset_state(ctx->state.as<DevState>(), ctx->gamma0, ctx->dw0, ctx->stream))) return rc;
+    if ((rc = launch_zero_i64(ctx->q_mov.as<int64_t>(), ctx->B, ctx->stream))) return rc;
+    ctx->q_dirty = false;
+    ctx->iter_host = 0;
+    ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return ERP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *erp_last_error(void) { return g_error.c_str(); }
+
+int erp_device_count(int *count) {
+    if (!count) return fail(ERP_ERR_INVALID, "null count");
+    hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess) { *count = 0; return fail(ERP_ERR_NO_DEVICE, hipGetErrorString(e)); }
+    return ERP_OK;
+}
+
+void erp_config_default(erp_config *cfg) {
+    if (!cfg) return;
+    cfg->num_bins_x = 512;            // placement.rs:50
+    cfg->num_bins_y = 512;
+    cfg->target_density = NAN;        // db.cell_utilization (placement.rs:134)
+    cfg->gamma_scale = 4.0;           // scheduler.rs:56
+    cfg->density_weight0 = 8e-5;      // scheduler.rs:57
+    cfg->spectral_mode = ERP_SPECTRAL_FFT;
+    cfg->device = 0;
+}
+
+int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
+    if (!out) return fail(ERP_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    int rc = validate(d, cfg);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(ERP_ERR_NO_DEVICE, "no HIP device");
+    if (cfg->device < 0 || cfg->device >= ndev) return fail(ERP_ERR_INVALID, "bad device ordinal");
+    if (cfg->spectral_mode != ERP_SPECTRAL_FFT && cfg->spectral_mode != ERP_SPECTRAL_MFMA_BF16X3)
+        return fail(ERP_ERR_UNSUPPORTED, "unknown spectral mode");
+
+    // ----------------------------------------------------- host-side index setup
+    const int64_t N = d->num_nodes, C = d->num_movable, P = d->num_pins_total, E = d->num_nets;
+    const float die_w = (float)d->die_w, die_h = (float)d->die_h;
+    for (int64_t i = 0; i < C; ++i)
+        if (!(die_w - d->w[i] - 1.f >= 0.f) || !(die_h - d->h[i] - 1.f >= 0.f))
+            return fail(ERP_ERR_INVALID, "movable instance " + std::to_string(i) +
+                                             " does not fit the die (W - w - 1 < 0)");
+    for (int64_t i = C; i < N; ++i)
+        if (!(d->x[i] >= 0.f) || !(d->y[i] >= 0.f))
+            return fail(ERP_ERR_INVALID, "fixed instance " + std::to_string(i) + " has a negative coordinate");
+    if (E > 0 && (d->net_start[0] != 0)) return fail(ERP_ERR_INVALID, "net_start[0] != 0");
+    std::vector<int32_t> slot_of_pin((size_t)P, -1);
+    std::vector<int64_t> order;                 // permuted net order
+    order.reserve((size_t)E);
+    for (int64_t e = 0; e < E; ++e) {
+        const int64_t a = d->net_start[e], b = d->net_start[e + 1];
+        if (b < a) return fail(ERP_ERR_INVALID, "net_start not monotonic");
+        if (b == a) return fail(ERP_ERR_INVALID, "net " + std::to_string(e) + " has no pins");
+        if (e < d->n_2nets && b - a != 2)
+            return fail(ERP_ERR_INVALID, "net " + std::to_string(e) + " below n_2nets is not two-pin");
+    }
+    for (int64_t e = 0; e < d->n_2nets; ++e) order.push_back(e);
+    std::vector<int64_t> multi;
+    for (int64_t e = d->n_2nets; e < E; ++e) multi.push_back(e);
+    std::stable_sort(multi.begin(), multi.end(), [&](int64_t a, int64_t b) {
+        return (d->net_start[a + 1] - d->net_start[a]) < (d->net_start[b + 1] - d->net_start[b]);
+    });
+    int64_t small_end = d->n_2nets;
+    for (int64_t e : multi) {
+        order.push_back(e);
+        if (d->net_start[e + 1] - d->net_start[e] <= kSmallDegree) ++small_end;
+    }
+    std::vector<int32_t> h_slot_node, h_net_start;
+    std::vector<float2> h_slot_off;
+    h_slot_node.reserve((size_t)P);
+    h_slot_off.reserve((size_t)P);
+    h_net_start.reserve((size_t)E + 1);
+    for (int64_t e : order) {
+        h_net_start.push_back((int32_t)h_slot_node.size());
+        for (int64_t k = d->net_start[e]; k < d->net_start[e + 1]; ++k) {
+            const int32_t pin = d->net_pins[k];
+            if (pin < 0 || pin >= P) return fail(ERP_ERR_INVALID, "pin id out of range");
+            if (slot_of_pin[pin] >= 0) return fail(ERP_ERR_INVALID, "pin " + std::to_string(pin) + " is in two nets");
+            const int32_t node = d->pin_node[pin];
+            if (node < 0 || node >= N) return fail(ERP_ERR_INVALID, "pin_node out of range");
+            slot_of_pin[pin] = (int32_t)h_slot_node.size();
+            h_slot_node.push_back(node);
+            h_slot_off.push_back(make_float2(d->pin_offset_x[pin], d->pin_offset_y[pin]));
+        }
+    }
+    h_net_start.push_back((int32_t)h_slot_node.size());
+    std::vector<int32_t> h_cstart((size_t)C + 1, 0), h_cslots;
+    for (int64_t c = 0; c < C; ++c) {
+        h_cstart[c] = (int32_t)h_cslots.size();
+        for (int64_t k = d->node_pin_start[c]; k < d->node_pin_start[c + 1]; ++k) {
+            const int32_t pin = d->node_pins[k];
+            if (pin < 0 || pin >= P || d->pin_node[pin] != c)
+                return fail(ERP_ERR_INVALID, "macro_2_pins inconsistent with pin_node");
+            if (slot_of_pin[pin] >= 0) h_cslots.push_back(slot_of_pin[pin]);
+        }
+    }
+    h_cstart[C] = (int32_t)h_cslots.size();
+
+    // ------------------------------------------------------------- scalars
+    erp_ctx *ctx = new erp_ctx();
+    ctx->device = cfg->device;
+    DeviceGuard g(ctx->device);
+    if (g.rc) { delete ctx; return g.rc; }
+    if (hipStreamCreate(&ctx->stream) != hipSuccess) {
+        delete ctx;
+        return fail(ERP_ERR_HIP, "hipStreamCreate failed");
+    }
+    hipStream_t s = ctx->stream;
+    ctx->N = N; ctx->C = C; ctx->F = N - C; ctx->P = P; ctx->E = E; ctx->n2 = d->n_2nets;
+    ctx->Pm = (int64_t)h_cslots.size();
+    ctx->m = cfg->num_bins_x; ctx->n = cfg->num_bins_y; ctx->B = (int64_t)ctx->m * ctx->n;
+    ctx->die_w = die_w; ctx->die_h = die_h;
+    ctx->bw = die_w / (float)ctx->m;                       // placement.rs:52-53
+    ctx->bh = die_h / (float)ctx->n;
+    ctx->t = std::isnan(cfg->target_density) ? (float)d->cell_utilization : (float)cfg->target_density;
+    ctx->movable_area = d->movable_area;
+    ctx->gamma0 = cfg->gamma_scale * ((double)ctx->bw + (double)ctx->bh);
+    ctx->dw0 = cfg->density_weight0;
+    ctx->spectral_mode = cfg->spectral_mode;
+    double mass = d->movable_area;
+    const double sq2bw = (double)(ctx->bw * 1.41421356237309504880f);
+    const double sq2bh = (double)(ctx->bh * 1.41421356237309504880f);
+    for (int64_t i = C; i < N; ++i)
+        mass += (double)ctx->t * std::max((double)d->w[i], sq2bw) * std::max((double)d->h[i], sq2bh);
+    ctx->fx_scale = fx_scale(mass * 1.01 + 1.0);
+    ctx->coef = (double)ctx->t * (double)ctx->bw * (double)ctx->bh;
+    ctx->coef_q = (int64_t)std::llrint(ctx->coef * ctx->fx_scale);
+
+    // ------------------------------------------------------------- device buffers
+#define TRY(x) do { rc = (x); if (rc) { erp_ctx_destroy(ctx); return rc; } } while (0)
+    TRY(upload(ctx->d_x, d->x, (size_t)N, s));
+    TRY(upload(ctx->d_y, d->y, (size_t)N, s));
+    TRY(upload(ctx->d_w, d->w, (size_t)N, s));
+    TRY(upload(ctx->d_h, d->h, (size_t)N, s));
+    TRY(upload(ctx->d_npins_all, d->num_pins, (size_t)N, s));
+    TRY(upload(ctx->slot_node, h_slot_node.data(), h_slot_node.size(), s));
+    TRY(upload(ctx->slot_off, h_slot_off.data(), h_slot_off.size(), s));
+    TRY(upload(ctx->net_start, h_net_start.data(), h_net_start.size(), s));
+    TRY(upload(ctx->cstart, h_cstart.data(), h_cstart.size(), s));
+    TRY(upload(ctx->cslots, h_cslots.data(), h_cslots.size(), s));
+    const size_t f2 = sizeof(float2);
+    TRY(ctx->pos.alloc(f2 * (size_t)N));
+    TRY(ctx->csize.alloc(f2 * (size_t)N));
+    TRY(ctx->coff.alloc(f2 * (size_t)N));
+    TRY(ctx->ratio.alloc(sizeof(float) * (size_t)N));
+    for (DevBuf *b : {&ctx->prev_pos, &ctx->prev_g, &ctx->g, &ctx->gwl, &ctx->gd, &ctx->hi})
+        TRY(b->alloc(f2 * (size_t)std::max<int64_t>(C, 1)));
+    TRY(ctx->area.alloc(sizeof(float) * (size_t)std::max<int64_t>(C, 1)));
+    TRY(ctx->npins.alloc(sizeof(float) * (size_t)std::max<int64_t>(C, 1)));
+    TRY(ctx->slot_grad.alloc(f2 * (size_t)std::max<int64_t>(P, 1)));
+    TRY(ctx->q_mov.alloc(sizeof(int64_t) * (size_t)ctx->B));
+    TRY(ctx->q_fix.alloc(sizeof(int64_t) * (size_t)ctx->B));
+    for (DevBuf *b : {&ctx->rho, &ctx->rowt, &ctx->p1, &ctx->p2, &ctx->p3, &ctx->fx, &ctx->fy,
+                      &ctx->pot, &ctx->auv})
+        TRY(b->alloc(sizeof(float) * (size_t)ctx->B));
+    TRY(ctx->state.alloc(sizeof(DevState)));
+    TRY(ctx->gamma_override.alloc(sizeof(double)));
+    TRY(ctx->hpwl_dbu.alloc(sizeof(unsigned long long)));
+
+    // ------------------------------------------------------------- kernel argument blocks
+    WaNets &w = ctx->wa;
+    w.slot_node = ctx->slot_node.as<int32_t>();
+    w.slot_off = ctx->slot_off.as<float2>();
+    w.net_start = ctx->net_start.as<int32_t>();
+    w.n2 = d->n_2nets;
+    w.small_begin = d->n_2nets;
+    w.small_end = small_end;
+    w.large_begin = small_end;
+    w.large_end = E;
+    w.slot_grad = ctx->slot_grad.as<float2>();
+    ctx->n_hpwl_part = wa_partials(w);
+    TRY(ctx->hpwl_part.alloc(sizeof(double) * (size_t)std::max(ctx->n_hpwl_part, 1)));
+    w.hpwl_part = ctx->hpwl_part.as<double>();
+
+    ctx->n_pot_part = potential_partials(ctx->m);
+    TRY(ctx->ovf_part.alloc(sizeof(int64_t) * (size_t)ctx->n_pot_part));
+    TRY(ctx->energy_part.alloc(sizeof(double) * (size_t)ctx->n_pot_part));
+    PotentialBufs &pb = ctx->pb;
+    pb.q_fix = ctx->q_fix.as<int64_t>();
+    pb.q_mov = ctx->q_mov.as<int64_t>();
+    pb.rho = ctx->rho.as<float>();
+    pb.rowt = ctx->rowt.as<float>();
+    pb.p1 = ctx->p1.as<float>();
+    pb.p2 = ctx->p2.as<float>();
+    pb.p3 = ctx->p3.as<float>();
+    pb.fx = ctx->fx.as<float>();
+    pb.fy = ctx->fy.as<float>();
+    pb.pot = ctx->pot.as<float>();
+    pb.auv = ctx->auv.as<float>();
+    pb.ovf_part = ctx->ovf_part.as<int64_t>();
+    pb.energy_part = ctx->energy_part.as<double>();
+    pb.m = ctx->m;
+    pb.n = ctx->n;
+    pb.inv_scale = 1.0 / ctx->fx_scale;
+    pb.coef_q = ctx->coef_q;
+    {
+        const float bin_area = ctx->bw * ctx->bh;           // epotential.rs:272
+        pb.inv_bin_area = 1.f / bin_area;                   // epotential.rs:284
+    }
+    pb.bw = ctx->bw;
+    pb.bh = ctx->bh;
+    pb.keep_q = false;
+
+    ctx->n_cell_part = cell_partials(C);
+    TRY(ctx->part_a.alloc(sizeof(double) * (size_t)ctx->n_cell_part));
+    TRY(ctx->part_b.alloc(sizeof(double) * (size_t)ctx->n_cell_part));
+    CellBufs &cb = ctx->cb;
+    cb.C = C;
+    cb.pos = ctx->pos.as<float2>();
+    cb.csize = ctx->csize.as<float2>();
+    cb.coff = ctx->coff.as<float2>();
+    cb.ratio = ctx->ratio.as<float>();
+    cb.area = ctx->area.as<float>();
+    cb.npins = ctx->npins.as<float>();
+    cb.hi = ctx->hi.as<float2>();
+    cb.cstart = ctx->cstart.as<int32_t>();
+    cb.cslots = ctx->cslots.as<int32_t>();
+    cb.slot_grad = ctx->slot_grad.as<float2>();
+    cb.gwl = ctx->gwl.as<float2>();
+    cb.gd = ctx->gd.as<float2>();
+    cb.g = ctx->g.as<float2>();
+    cb.prev_pos = ctx->prev_pos.as<float2>();
+    cb.prev_g = ctx->prev_g.as<float2>();
+    cb.pos_out = ctx->pos.as<float2>();
+    cb.part_a = ctx->part_a.as<double>();
+    cb.part_b = ctx->part_b.as<double>();
+    cb.fx = ctx->fx.as<float>();
+    cb.fy = ctx->fy.as<float>();
+    cb.m = ctx->m;
+    cb.n = ctx->n;
+    cb.bw = ctx->bw;
+    cb.bh = ctx->bh;
+
+    ScalarArgs &sa = ctx->sa;
+    sa.st = ctx->state.as<DevState>();
+    sa.hpwl_part = ctx->hpwl_part.as<double>();
+    sa.n_hpwl = ctx->n_hpwl_part;
+    sa.ovf_part = ctx->ovf_part.as<int64_t>();
+    sa.n_ovf = ctx->n_pot_part;
+    sa.energy_part = ctx->energy_part.as<double>();
+    sa.n_energy = ctx->n_pot_part;
+    sa.part_a = ctx->part_a.as<double>();
+    sa.part_b = ctx->part_b.as<double>();
+    sa.n_cell = ctx->n_cell_part;
+    sa.inv_scale_over_area = 1.0 / (ctx->fx_scale * ctx->movable_area);
+
+    // ------------------------------------------------------------- device setup kernels
+    for (DevBuf *b : {&ctx->hpwl_part, &ctx->ovf_part, &ctx->energy_part, &ctx->part_a, &ctx->part_b})
+        if (hipMemsetAsync(b->p, 0, b->bytes, s) != hipSuccess) { erp_ctx_destroy(ctx); return fail(ERP_ERR_HIP, "hipMemsetAsync failed"); }
+    TRY(launch_setup_cells(ctx->d_w.as<float>(), ctx->d_h.as<float>(), ctx->d_npins_all.as<float>(),
+                           N, C, ctx->bw, ctx->bh, ctx->t, die_w, die_h, ctx->csize.as<float2>(),
+                           ctx->coff.as<float2>(), ctx->ratio.as<float>(), ctx->area.as<float>(),
+                           ctx->npins.as<float>(), ctx->hi.as<float2>(), s));
+    TRY(reset_ctx(ctx));
+    TRY(launch_zero_i64(ctx->q_fix.as<int64_t>(), ctx->B, s));
+    TRY(launch_fixed_map(ctx->pos.as<float2>() + C, ctx->csize.as<float2>() + C, N - C, ctx->m,
+                         ctx->n, ctx->bw, ctx->bh, die_w, die_h, ctx->fx_scale,
+                         ctx->q_fix.as<int64_t>(), s));
+    TRY(launch_scale_fixed(ctx->q_fix.as<int64_t>(), ctx->B, (double)ctx->t, s));
+    TRY(make_spec_tables(ctx->tm, ctx->m, s));
+    TRY(make_spec_tables(ctx->tn, ctx->n, s));
+    if (ctx->spectral_mode == ERP_SPECTRAL_MFMA_BF16X3)
+        TRY(mfma_plan_create(&ctx->mfma, ctx->m, ctx->n, s));
+    ctx->timer = new KernelTimer(s);
+    if (hipStreamSynchronize(s) != hipSuccess) { erp_ctx_destroy(ctx); return fail(ERP_ERR_HIP, "setup failed"); }
+#undef TRY
+    *out = ctx;
+    return ERP_OK;
+}
+
+int erp_ctx_destroy(erp_ctx *ctx) {
+    if (!ctx) return ERP_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    drop_graphs(ctx);
+    if (ctx->mfma) mfma_plan_destroy(ctx->mfma);
+    delete ctx->timer;
+    hipStream_t s = ctx->stream;
+    delete ctx;   // releases every DevBuf
+    if (s) (void)hipStreamDestroy(s);
+    return ERP_OK;
+}
+
+int erp_ctx_reset(erp_ctx *ctx) {
+    if (!ctx) return fail(ERP_ERR_INVALID, "null context");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    return reset_ctx(ctx);
+}
+
+int erp_set_positions(erp_ctx *ctx, const float *x, const float *y) {
+    if (!ctx || !x || !y) return fail(ERP_ERR_INVALID, "null argument");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    std::vector<float2> p((size_t)ctx->N);
+    for (int64_t i = 0; i < ctx->N; ++i) p[i] = make_float2(x[i], y[i]);
+    ERP_HIP_TRY(hipMemcpyAsync(ctx->pos.p, p.data(), sizeof(float2) * p.size(), hipMemcpyHostToDevice, ctx->stream));
+    ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return ERP_OK;
+}
+
+int erp_get_positions(erp_ctx *ctx, float *x, float *y) {
+    if (!ctx || !x || !y) return fail(ERP_ERR_INVALID, "null argument");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    std::vector<float2> p((size_t)ctx->N);
+    ERP_HIP_TRY(hipMemcpyAsync(p.data(), ctx->pos.p, sizeof(float2) * p.size(), hipMemcpyDeviceToHost, ctx->stream));
+    ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i < ctx->N; ++i) { x[i] = p[i].x; y[i] = p[i].y; }
+    return ERP_OK;
+}
+
+static int read_scalar(erp_ctx *ctx, size_t offset, double *out) {
+    ERP_HIP_TRY(hipMemcpyAsync(out, (char *)ctx->state.p + offset, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return ERP_OK;
+}
+
+int erp_compute_hpwl(erp_ctx *ctx, double *hpwl) {
+    if (!ctx) return fail(ERP_ERR_INVALID, "null context");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    int rc = phase_hpwl(ctx, &ctx->state.as<DevState>()->gamma);
+    if (rc || !hpwl) return rc;
+    return read_scalar(ctx, offsetof(DevState, hpwl), hpwl);
+}
+
+int erp_compute_hpwl_gamma(erp_ctx *ctx, double gamma, double *hpwl) {
+    if (!ctx) return fail(ERP_ERR_INVALID, "null context");
+    if (!(gamma > 0)) return fail(ERP_ERR_INVALID, "gamma must be positive");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    ERP_HIP_TRY(hipMemcpyAsync(ctx->gamma_override.p, &gamma, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    int rc = phase_hpwl(ctx, ctx->gamma_override.as<double>());
+    if (rc || !hpwl) return rc;
+    return read_scalar(ctx, offsetof(DevState, hpwl), hpwl);
+}
+
+int erp_compute_potential(erp_ctx *ctx, double *energy, double *overflow) {
+    if (!ctx) return fail(ERP_ERR_INVALID, "null context");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    int rc = phase_potential(ctx);
+    if (rc) return rc;
+    if (energy && (rc = read_scalar(ctx, offsetof(DevState, energy), energy))) return rc;
+    if (overflow && (rc = read_scalar(ctx, offsetof(DevState, overflow), overflow))) return rc;
+    return ERP_OK;
+}
+
+int erp_nesterov_step(erp_ctx *ctx) {
+    if (!ctx) return fail(ERP_ERR_INVALID, "null context");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    return phase_nesterov(ctx);
+}
+
+int erp_scheduler_update(erp_ctx *ctx) {
+    if (!ctx) return fail(ERP_ERR_INVALID, "null context");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    return phase_scheduler(ctx);
+}
+
+int erp_step(erp_ctx *ctx, erp_step_info *info) {
+    if (!ctx) return fail(ERP_ERR_INVALID, "null context");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    if (ctx->timing) ctx->timer->clear();
+    int rc = phase_hpwl(ctx, &ctx->state.as<DevState>()->gamma);
+    if (!rc) rc = phase_potential(ctx);
+    if (!rc) rc = phase_nesterov(ctx);
+    if (!rc) rc = phase_scheduler(ctx);
+    if (rc) return rc;
+    if (ctx->timing) { if ((rc = ctx->timer->collect())) return rc; }
+    return read_info(ctx, info);
+}
+
+int erp_run(erp_ctx *ctx, int64_t iterations, erp_step_info *info) {
+    if (!ctx) return fail(ERP_ERR_INVALID, "null context");
+    if (iterations < 0) return fail(ERP_ERR_INVALID, "negative iteration count");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    int rc = ERP_OK;
+    if (ctx->timing) ctx->timer->clear();
+    while (iterations > 0 && ctx->iter_host == 0) {   // iteration 0: density-weight scaling
+        rc = phase_hpwl(ctx, &ctx->state.as<DevState>()->gamma);
+        if (!rc) rc = phase_potential(ctx);
+        if (!rc) rc = phase_nesterov(ctx);
+        if (!rc) rc = phase_scheduler(ctx);
+        if (rc) return rc;
+        --iterations;
+    }
+    if (ctx->timing || ctx->keep_q) {
+        for (int64_t k = 0; k < iterations && !rc; ++k) {
+            rc = launch_fused_iteration(ctx);
+            ctx->iter_host += 1;
+            if (!rc && ctx->timing && (k % 16 == 15)) rc = ctx->timer->collect();
+        }
+        if (!rc && ctx->timing) rc = ctx->timer->collect();
+    } else if (iterations > 0) {
+        if (!ctx->graph_many && (rc = capture(ctx, kGraphIters, &ctx->graph_many))) return rc;
+        if (!ctx->graph_one && (rc = capture(ctx, 1, &ctx->graph_one))) return rc;
+        int64_t k = 0;
+        for (; k + kGraphIters <= iterations; k += kGraphIters) {
+            ERP_HIP_TRY(hipGraphLaunch(ctx->graph_many, ctx->stream));
+            ctx->iter_host += kGraphIters;
+        }
+        for (; k < iterations; ++k) {
+            ERP_HIP_TRY(hipGraphLaunch(ctx->graph_one, ctx->stream));
+            ctx->iter_host += 1;
+        }
+    }
+    if (rc) return rc;
+    if (info) return read_info(ctx, info);
+    ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return ERP_OK;
+}
+
+int erp_run_until(erp_ctx *ctx, int64_t max_iterations, double stop_overflow, int32_t check_every,
+                  erp_step_info *info) {
+    if (!ctx) return fail(ERP_ERR_INVALID, "null context");
+    if (check_every <= 0) check_every = 1;
+    erp_step_info tmp{};
+    int64_t done = 0;
+    while (done < max_iterations) {
+        const int64_t chunk = std::min<int64_t>(check_every, max_iterations - done);
+        int rc = erp_run(ctx, chunk, &tmp);
+        if (rc) return rc;
+        done += chunk;
+        if (tmp.overflow < stop_overflow || tmp.nonfinite) break;
+    }
+    if (info) *info = tmp;
+    return ERP_OK;
+}
+
+int erp_get_step_info(erp_ctx *ctx, erp_step_info *info) {
+    if (!ctx || !info) return fail(ERP_ERR_INVALID, "null argument");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    return read_info(ctx, info);
+}
+
+int erp_get_map(erp_ctx *ctx, int which, float *out) {
+    if (!ctx || !out) return fail(ERP_ERR_INVALID, "null argument");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (which == ERP_MAP_INITIAL) {
+        std::vector<int64_t> q((size_t)ctx->B);
+        ERP_HIP_TRY(hipMemcpy(q.data(), ctx->q_fix.p, sizeof(int64_t) * q.size(), hipMemcpyDeviceToHost));
+        for (int64_t b = 0; b < ctx->B; ++b) out[b] = (float)((double)q[b] / ctx->fx_scale);
+        return ERP_OK;
+    }
+    const DevBuf *src = which == ERP_MAP_DENSITY ? &ctx->rho : which == ERP_MAP_AUV ? &ctx->auv
+                      : which == ERP_MAP_FIELD_X ? &ctx->fx : which == ERP_MAP_FIELD_Y ? &ctx->fy
+                      : which == ERP_MAP_POTENTIAL ? &ctx->pot : nullptr;
+    if (!src) return fail(ERP_ERR_INVALID, "unknown map");
+    ERP_HIP_TRY(hipMemcpy(out, src->p, sizeof(float) * (size_t)ctx->B, hipMemcpyDeviceToHost));
+    return ERP_OK;
+}
+
+int erp_get_cell_grads(erp_ctx *ctx, int which, float *gx, float *gy) {
+    if (!ctx || !gx || !gy) return fail(ERP_ERR_INVALID, "null argument");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    const DevBuf *src = which == ERP_GRAD_WIRELENGTH ? &ctx->gwl : which == ERP_GRAD_DENSITY ? &ctx->gd
+                      : which == ERP_GRAD_COMBINED ? &ctx->g : nullptr;
+    if (!src) return fail(ERP_ERR_INVALID, "unknown gradient");
+    ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::vector<float2> v((size_t)ctx->C);
+    if (ctx->C) ERP_HIP_TRY(hipMemcpy(v.data(), src->p, sizeof(float2) * v.size(), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < ctx->C; ++i) { gx[i] = v[i].x; gy[i] = v[i].y; }
+    return ERP_OK;
+}
+
+int erp_get_density_fixed(erp_ctx *ctx, int64_t *q_mov, int64_t *q_fix, double *scale) {
+    if (!ctx) return fail(ERP_ERR_INVALID, "null context");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (q_mov) {
+        if (!ctx->q_dirty) return fail(ERP_ERR_INVALID, "movable accumulators were cleared; enable keep mode with erp_set_keep_density");
+        ERP_HIP_TRY(hipMemcpy(q_mov, ctx->q_mov.p, sizeof(int64_t) * (size_t)ctx->B, hipMemcpyDeviceToHost));
+    }
+    if (q_fix) ERP_HIP_TRY(hipMemcpy(q_fix, ctx->q_fix.p, sizeof(int64_t) * (size_t)ctx->B, hipMemcpyDeviceToHost));
+    if (scale) *scale = ctx->fx_scale;
+    return ERP_OK;
+}
+
+int erp_set_keep_density(erp_ctx *ctx, int keep) {
+    if (!ctx) return fail(ERP_ERR_INVALID, "null context");
+    ctx->keep_q = keep != 0;
+    return ERP_OK;
+}
+
+int erp_set_store_gradients(erp_ctx *ctx, int store) {
+    if (!ctx) return fail(ERP_ERR_INVALID, "null context");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    if (ctx->debug_parts != (store != 0)) drop_graphs(ctx);
+    ctx->debug_parts = store != 0;
+    return ERP_OK;
+}
+
+int erp_get_setup(erp_ctx *ctx, float *cw, float *ch, float *ox, float *oy, float *ratio,
+                  double *scalars) {
+    if (!ctx) return fail(ERP_ERR_INVALID, "null context");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::vector<float2> a((size_t)ctx->N), b((size_t)ctx->N);
+    ERP_HIP_TRY(hipMemcpy(a.data(), ctx->csize.p, sizeof(float2) * a.size(), hipMemcpyDeviceToHost));
+    ERP_HIP_TRY(hipMemcpy(b.data(), ctx->coff.p, sizeof(float2) * b.size(), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < ctx->N; ++i) {
+        if (cw) cw[i] = a[i].x;
+        if (ch) ch[i] = a[i].y;
+        if (ox) ox[i] = b[i].x;
+        if (oy) oy[i] = b[i].y;
+    }
+    if (ratio) ERP_HIP_TRY(hipMemcpy(ratio, ctx->ratio.p, sizeof(float) * (size_t)ctx->N, hipMemcpyDeviceToHost));
+    if (scalars) {
+        scalars[0] = ctx->bw;
+        scalars[1] = ctx->bh;
+        scalars[2] = ctx->t;
+        scalars[3] = ctx->fx_scale;
+    }
+    return ERP_OK;
+}
+
+int erp_final_hpwl_dbu(erp_ctx *ctx, int64_t *hpwl_dbu) {
+    if (!ctx || !hpwl_dbu) return fail(ERP_ERR_INVALID, "null argument");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    unsigned long long zero = 0, v = 0;
+    ERP_HIP_TRY(hipMemcpyAsync(ctx->hpwl_dbu.p, &zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
+    int rc = launch_hpwl_dbu(ctx->wa, ctx->pos.as<float2>(), ctx->E, ctx->hpwl_dbu.as<unsigned long long>(), ctx->stream);
+    if (rc) return rc;
+    ERP_HIP_TRY(hipMemcpyAsync(&v, ctx->hpwl_dbu.p, sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
+    ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *hpwl_dbu = (int64_t)v;
+    return ERP_OK;
+}
+
+int erp_set_timing(erp_ctx *ctx, int enabled) {
+    if (!ctx) return fail(ERP_ERR_INVALID, "null context");
+    ctx->timing = enabled != 0;
+    if (ctx->timer) ctx->timer->reset_totals();
+    return ERP_OK;
+}
+
+int erp_get_timing(erp_ctx *ctx, double *ms, int32_t capacity, int32_t *count, const char **names) {
+    if (!ctx || !count) return fail(ERP_ERR_INVALID, "null argument");
+    return timer_kernel_names(ctx, ms, capacity, count, names);
+}
+
+// ------------------------------------------------------------------ leaf kernels
+namespace {
+struct LeafScope {
+    hipStream_t s = nullptr;
+    int rc = ERP_OK;
+    LeafScope() {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) { rc = fail(ERP_ERR_NO_DEVICE, "no HIP device"); return; }
+        if (hipStreamCreate(&s) != hipSuccess) rc = fail(ERP_ERR_HIP, "hipStreamCreate failed");
+    }
+    ~LeafScope() { if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); } }
+};
+
+int leaf_spectral(const float *x, float *out, int32_t m, int32_t n, int32_t mode, int which) {
+    if (!x || !out) return fail(ERP_ERR_INVALID, "null argument");
+    if (!check_grid(m) || !check_grid(n))
+        return fail(ERP_ERR_UNSUPPORTED, "sizes must be powers of two in [64, 2048]");
+    if (mode != ERP_SPECTRAL_FFT) return fail(ERP_ERR_UNSUPPORTED, "leaf transforms run the FFT form");
+    LeafScope L;
+    if (L.rc) return L.rc;
+    const size_t B = (size_t)m * n;
+    DevBuf dx, dout, dscr;
+    SpecTables tm, tn;
+    int rc;
+    if ((rc = upload(dx, x, B, L.s)) || (rc = dout.alloc(sizeof(float) * B)) ||
+        (rc = dscr.alloc(sizeof(float) * B)) || (rc = make_spec_tables(tm, m, L.s)) ||
+        (rc = make_spec_tables(tn, n, L.s)))
+        return rc;
+    const float *px = dx.as<float>();
+    float *po = dout.as<float>(), *ps = dscr.as<float>();
+    switch (which) {
+    case 0: rc = launch_dct2(px, po, ps, m, n, tm, tn, L.s); break;
+    case 1: rc = launch_inverse2(px, po, ps, m, n, INV_IDCT, INV_IDCT, tm, tn, L.s); break;
+    case 2: rc = launch_inverse2(px, po, ps, m, n, INV_IDCT, INV_IDXST, tm, tn, L.s); break;
+    default: rc = launch_inverse2(px, po, ps, m, n, INV_IDXST, INV_IDCT, tm, tn, L.s); break;
+    }
+    if (rc) return rc;
+    ERP_HIP_TRY(hipMemcpyAsync(out, po, sizeof(float) * B, hipMemcpyDeviceToHost, L.s));
+    ERP_HIP_TRY(hipStreamSynchronize(L.s));
+    return ERP_OK;
+}
+
+void pack2(std::vector<float2> &v, const float *a, const float *b, int64_t n) {
+    v.resize((size_t)n);
+    for (int64_t i = 0; i < n; ++i) v[i] = make_float2(a[i], b[i]);
+}
+}  // namespace
+
+int erp_dct2_fft2_forward(const float *x, float *out, int32_t m, int32_t n, int32_t mode) {
+    return leaf_spectral(x, out, m, n, mode, 0);
+}
+int erp_idct2_fft2_forward(const float *x, float *out, int32_t m, int32_t n, int32_t mode) {
+    return leaf_spectral(x, out, m, n, mode, 1);
+}
+int erp_idct_idxst_forward(const float *x, float *out, int32_t m, int32_t n, int32_t mode) {
+    return leaf_spectral(x, out, m, n, mode, 2);
+}
+int erp_idxst_idct_forward(const float *x, float *out, int32_t m, int32_t n, int32_t mode) {
+    return leaf_spectral(x, out, m, n, mode, 3);
+}
+
+int erp_compute_density_map(int64_t nc, const float *x, const float *y, const float *cw,
+                            const float *ch, const float *ox, const float *oy, const float *ratio,
+                            int32_t m, int32_t n, float bw, float bh, double scale,
+                            const float *initial, float *map, int64_t *q_out) {
+    if (nc < 0 || (nc > 0 && (!x || !y || !cw || !ch || !ox || !oy || !ratio)) || !map)
+        return fail(ERP_ERR_INVALID, "null argument");
+    if (m < 2 || n < 2 || !(bw > 0) || !(bh > 0) || !(scale > 0))
+        return fail(ERP_ERR_INVALID, "bad grid, bin size or scale");
+    LeafScope L;
+    if (L.rc) return L.rc;
+    const size_t B = (size_t)m * n;
+    std::vector<float2> p, sz, of;
+    pack2(p, x, y, nc);
+    pack2(sz, cw, ch, nc);
+    pack2(of, ox, oy, nc);
+    DevBuf dp, ds, dof, dr, dq, dinit, dmap;
+    int rc;
+    if ((rc = upload(dp, p.data(), p.size(), L.s)) || (rc = upload(ds, sz.data(), sz.size(), L.s)) ||
+        (rc = upload(dof, of.data(), of.size(), L.s)) || (rc = upload(dr, ratio, (size_t)nc, L.s)) ||
+        (rc = dq.alloc(sizeof(int64_t) * B)) || (rc = dmap.alloc(sizeof(float) * B)))
+        return rc;
+    if (initial && (rc = upload(dinit, initial, B, L.s))) return rc;
+    if ((rc = launch_zero_i64(dq.as<int64_t>(), (int64_t)B, L.s))) return rc;
+    if ((rc = launch_scatter(dp.as<float2>(), ds.as<float2>(), dof.as<float2>(), dr.as<float>(), nc,
+                             m, n, bw, bh, scale, dq.as<int64_t>(), L.s)))
+        return rc;
+    if ((rc = launch_finalize_leaf(dq.as<int64_t>(), initial ? dinit.as<float>() : nullptr,
+                                   dmap.as<float>(), (int64_t)B, 1.0 / scale, L.s)))
+        return rc;
+    ERP_HIP_TRY(hipMemcpyAsync(map, dmap.p, sizeof(float) * B, hipMemcpyDeviceToHost, L.s));
+    if (q_out) ERP_HIP_TRY(hipMemcpyAsync(q_out, dq.p, sizeof(int64_t) * B, hipMemcpyDeviceToHost, L.s));
+    ERP_HIP_TRY(hipStreamSynchronize(L.s));
+    return ERP_OK;
+}
+
+int erp_compute_initial_density_map(int64_t nf, const float *x, const float *y, const float *cw,
+                                    const float *ch, int32_t m, int32_t n, float bw, float bh,
+                                    float die_w, float die_h, double scale, float *map,
+                                    int64_t *q_out) {
+    if (nf < 0 || (nf > 0 && (!x || !y || !cw || !ch)) || !map)
+        return fail(ERP_ERR_INVALID, "null argument");
+    if (m < 2 || n < 2 || !(bw > 0) || !(bh > 0) || !(scale > 0))
+        return fail(ERP_ERR_INVALID, "bad grid, bin size or scale");
+    for (int64_t i = 0; i < nf; ++i)
+        if (!(x[i] >= 0.f) || !(y[i] >= 0.f)) return fail(ERP_ERR_INVALID, "negative fixed-cell coordinate");
+    LeafScope L;
+    if (L.rc) return L.rc;
+    const size_t B = (size_t)m * n;
+    std::vector<float2> p, sz;
+    pack2(p, x, y, nf);
+    pack2(sz, cw, ch, nf);
+    DevBuf dp, ds, dq, dmap;
+    int rc;
+    if ((rc = upload(dp, p.data(), p.size(), L.s)) || (rc = upload(ds, sz.data(), sz.size(), L.s)) ||
+        (rc = dq.alloc(sizeof(int64_t) * B)) || (rc = dmap.alloc(sizeof(float) * B)))
+        return rc;
+    if ((rc = launch_zero_i64(dq.as<int64_t>(), (int64_t)B, L.s))) return rc;
+    if ((rc = launch_fixed_map(dp.as<float2>(), ds.as<float2>(), nf, m, n, bw, bh, die_w, die_h,
+                               scale, dq.as<int64_t>(), L.s)))
+        return rc;
+    if ((rc = launch_finalize_leaf(dq.as<int64_t>(), nullptr, dmap.as<float>(), (int64_t)B,
+                                   1.0 / scale, L.s)))
+        return rc;
+    ERP_HIP_TRY(hipMemcpyAsync(map, dmap.p, sizeof(float) * B, hipMemcpyDeviceToHost, L.s));
+    if (q_out) ERP_HIP_TRY(hipMemcpyAsync(q_out, dq.p, sizeof(int64_t) * B, hipMemcpyDeviceToHost, L.s));
+    ERP_HIP_TRY(hipStreamSynchronize(L.s));
+    return ERP_OK;
+}
+
+int erp_compute_electric_force(int64_t nc, int32_t m, int32_t n, const float *field_x,
+                               const float *field_y, const float *x, const float *y,
+                               const float *cw, const float *ch, const float *ox, const float *oy,
+                               const float *ratio, float bw, float bh, float *grad_x,
+                               float *grad_y) {
+    if (nc < 0 || !field_x || !field_y || (nc > 0 && (!x || !y || !cw || !ch || !ox || !oy ||
+                                                      !ratio || !grad_x || !grad_y)))
+        return fail(ERP_ERR_INVALID, "null argument");
+    if (m < 1 || n < 1 || !(bw > 0) || !(bh > 0)) return fail(ERP_ERR_INVALID, "bad grid");
+    if (nc == 0) return ERP_OK;
+    LeafScope L;
+    if (L.rc) return L.rc;
+    const size_t B = (size_t)m * n;
+    std::vector<float2> p, sz, of;
+    pack2(p, x, y, nc);
+    pack2(sz, cw, ch, nc);
+    pack2(of, ox, oy, nc);
+    DevBuf dp, ds, dof, dr, dfx, dfy, dg;
+    int rc;
+    if ((rc = upload(dp, p.data(), p.size(), L.s)) || (rc = upload(ds, sz.data(), sz.size(), L.s)) ||
+        (rc = upload(dof, of.data(), of.size(), L.s)) || (rc = upload(dr, ratio, (size_t)nc, L.s)) ||
+        (rc = upload(dfx, field_x, B, L.s)) || (rc = upload(dfy, field_y, B, L.s)) ||
+        (rc = dg.alloc(sizeof(float2) * (size_t)nc)))
+        return rc;
+    CellBufs c{};
+    c.C = nc;
+    c.pos = dp.as<float2>();
+    c.csize = ds.as<float2>();
+    c.coff = dof.as<float2>();
+    c.ratio = dr.as<float>();
+    c.gd = dg.as<float2>();
+    c.fx = dfx.as<float>();
+    c.fy = dfy.as<float>();
+    c.m = m;
+    c.n = n;
+    c.bw = bw;
+    c.bh = bh;
+    if ((rc = launch_efield(c, L.s))) return rc;
+    std::vector<float2> gv((size_t)nc);
+    ERP_HIP_TRY(hipMemcpyAsync(gv.data(), dg.p, sizeof(float2) * gv.size(), hipMemcpyDeviceToHost, L.s));
+    ERP_HIP_TRY(hipStreamSynchronize(L.s));
+    for (int64_t i = 0; i < nc; ++i) { grad_x[i] = gv[i].x; grad_y[i] = gv[i].y; }
+    return ERP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,130 @@
+// Internal declarations shared by the HIP translation units of liberp_gpu.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/erp_gpu.h"
+
+namespace erp {
+
+struct KernelTimer;
+
+// ------------------------------------------------------------------ errors
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define ERP_HIP_TRY(expr)                                                              \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess)                                                           \
+            return ::erp::fail(ERP_ERR_HIP, std::string(#expr " failed: ") +           \
+                                                hipGetErrorString(_e));                 \
+    } while (0)
+
+// ---------------------------------------------------------- device buffers
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    int alloc(size_t nbytes) {
+        release();
+        if (nbytes == 0) nbytes = 16;
+        hipError_t e = hipMalloc(&p, nbytes);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(ERP_ERR_HIP, std::string("hipMalloc(") + std::to_string(nbytes) +
+                                         ") failed: " + hipGetErrorString(e));
+        }
+        bytes = nbytes;
+        return ERP_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// --------------------------------------------------------------- constants
+constexpr int kBlock = 256;          // threads per workgroup for streaming kernels
+constexpr int kMinBins = 64, kMaxBins = 2048;
+
+inline bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+inline int ilog2(int64_t v) { int r = 0; while ((int64_t(1) << r) < v) ++r; return r; }
+
+// Device-resident scalar state of one placement (Scheduler + Nesterov + the
+// reductions of the current step).  FP64 throughout.
+struct DevState {
+    // Scheduler (scheduler.rs:36-46)
+    int64_t iteration;
+    double gamma, base_gamma, density_weight;
+    int32_t hpwl_has_initial, ovf_has_initial;
+    double hpwl_initial, hpwl_cur, hpwl_prev;
+    double ovf_initial, ovf_cur, ovf_prev;
+    // Nesterov (nesterov.rs:6-13)
+    double alpha, beta;
+    // density weight the current step's Nesterov update used
+    double dw_used;
+    // reductions of the current step
+    double hpwl, overflow, energy, l1_wl, l1_d, num, den;
+    int32_t nonfinite, should_stop;
+};
+
+// Spectral transform tables for one length L (power of two).
+struct SpecTables {
+    int L = 0;
+    DevBuf fft_tw;   // float2[L/2]: e^{-2 pi i k / L}
+    DevBuf dct_tw;   // float2[L]:   e^{-i pi k / (2L)}
+};
+
+// ----------------------------------------------------------- launch helpers
+int make_spec_tables(SpecTables &t, int L, hipStream_t s);
+
+// Spectral operators (erp_spectral.hip).  All buffers are device float[m*n].
+// Kinds of one-dimensional inverse transforms:
+enum InvKind { INV_IDCT = 0, INV_IDXST = 1 };
+
+// a = dctn(x, 2) / (m n)     (scratch: float[m*n])
+int launch_dct2(const float *x, float *out, float *scratch, int m, int n,
+                const SpecTables &tm, const SpecTables &tn, hipStream_t s);
+// out = inverse along x (kind_x) then along y (kind_y); scale 4
+int launch_inverse2(const float *x, float *out, float *scratch, int m, int n, InvKind kind_x,
+                    InvKind kind_y, const SpecTables &tm, const SpecTables &tn, hipStream_t s);
+
+// Fused potential pipeline (FFT form).
+struct PotentialBufs {
+    const int64_t *q_fix;   // [B]
+    int64_t *q_mov;         // [B]  read and cleared by pass 1
+    float *rho;             // [B]  density map after 1/bin_area scaling
+    float *rowt;            // [B]  row-DCT intermediate
+    float *p1, *p2, *p3;    // [B]  column-inverse intermediates
+    float *fx, *fy, *pot;   // [B]
+    float *auv;             // [B] or nullptr (store DCT coefficients)
+    int64_t *ovf_part;      // [m / rows-per-block] int64 overflow partials
+    double *energy_part;    // [m / rows-per-block] FP64 energy partials
+    int m, n;
+    double inv_scale;       // 1/S of the fixed-point accumulators
+    int64_t coef_q;         // round(target_density * bin_area * S)
+    float inv_bin_area;     // T::one() / bin_area in f32
+    float bw, bh;
+    bool keep_q;            // leave q_mov intact (parity inspection)
+};
+int potential_partials(int m);   // number of partials pass 1 / pass 3 write
+int launch_potential_fft(const PotentialBufs &b, const SpecTables &tm, const SpecTables &tn,
+                         hipStream_t s, KernelTimer *t = nullptr);
+// Potential with MFMA dense transforms (erp_mfma.hip).
+struct MfmaPlan;
+int mfma_plan_create(MfmaPlan **plan, int m, int n, hipStream_t s);
+void mfma_plan_destroy(MfmaPlan *plan);
+int launch_potential_mfma(const PotentialBufs &b, MfmaPlan *plan, hipStream_t s,
+                          KernelTimer *t = nullptr);
+
+}  // namespace erp

@@ -1,0 +1,361 @@
+// Per-cell kernels (field gather, WA pin->cell gather, preconditioned Nesterov
+// step) and the device-side scalar state (Nesterov momentum/step size and the
+// Scheduler), gfx950.
+#include "erp_internal.h"
+#include "erp_kernels.h"
+
+namespace erp {
+namespace {
+
+__device__ __forceinline__ double block_sum_f64(double v, double *red) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// eforce.rs:6-12: overlap of [x, x+s] with bin k, NOT clamped at 0
+__device__ __forceinline__ float tri_unclamped(float x, float s, int k, float bs) {
+    const float bin_xl = 0.f + (float)k * bs;
+    return fminf(x + s, bin_xl + bs) - fmaxf(x, bin_xl);
+}
+
+// eforce.rs:40-90 for one cell: -ratio * sum_b overlap_x overlap_y field_b,
+// bins [max(trunc(x/bw), 0), min(trunc((x+s)/bw) + 1, M)).
+__device__ __forceinline__ float2 efield_cell(const CellBufs &c, int64_t i, float inv_bw,
+                                              float inv_bh) {
+    const float2 p = c.pos[i], o = c.coff[i], sz = c.csize[i];
+    const float nx = p.x + o.x, ny = p.y + o.y;
+    int xl = (int)((nx - 0.f) * inv_bw);
+    xl = xl < 0 ? 0 : xl;
+    int xh = (int)((nx + sz.x - 0.f) * inv_bw) + 1;
+    xh = xh > c.m ? c.m : xh;
+    int yl = (int)((ny - 0.f) * inv_bh);
+    yl = yl < 0 ? 0 : yl;
+    int yh = (int)((ny + sz.y - 0.f) * inv_bh) + 1;
+    yh = yh > c.n ? c.n : yh;
+    float gx = 0.f, gy = 0.f;
+    for (int k = xl; k < xh; ++k) {
+        const float px = tri_unclamped(nx, sz.x, k, c.bw);
+        const float *fxr = c.fx + (size_t)k * c.n;
+        const float *fyr = c.fy + (size_t)k * c.n;
+        for (int h = yl; h < yh; ++h) {
+            const float py = tri_unclamped(ny, sz.y, h, c.bh);
+            const float area = px * py;
+            gx += area * fxr[h];
+            gy += area * fyr[h];
+        }
+    }
+    const float r = c.ratio[i];
+    return make_float2(-gx * r, -gy * r);
+}
+
+// hpwl.rs:120-134: sum of the cell's pin gradients in macro_2_pins order
+__device__ __forceinline__ float2 pin2cell(const CellBufs &c, int64_t i) {
+    float gx = 0.f, gy = 0.f;
+    for (int k = c.cstart[i]; k < c.cstart[i + 1]; ++k) {
+        const float2 g = c.slot_grad[c.cslots[k]];
+        gx += g.x;
+        gy += g.y;
+    }
+    return make_float2(gx, gy);
+}
+
+// nesterov.rs:75-95: preconditioned combined gradient
+__device__ __forceinline__ float2 combine(const CellBufs &c, int64_t i, float2 gwl, float2 gd,
+                                          float dw) {
+    const float inv_p = 1.f / fminf(c.npins[i] + dw * c.area[i], 1.f);
+    return make_float2((gwl.x + dw * gd.x) * inv_p, (gwl.y + dw * gd.y) * inv_p);
+}
+
+__global__ void __launch_bounds__(kBlock) k_pin2cell(CellBufs c) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < c.C) c.gwl[i] = pin2cell(c, i);
+}
+
+__global__ void __launch_bounds__(kBlock) k_efield(CellBufs c) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < c.C) c.gd[i] = efield_cell(c, i, 1.f / c.bw, 1.f / c.bh);
+}
+
+__global__ void __launch_bounds__(kBlock) k_l1(CellBufs c) {
+    __shared__ double red[kBlock];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0, b = 0.0;
+    if (i < c.C) {
+        const float2 w = c.gwl[i], d = c.gd[i];
+        a = (double)fabsf(w.x) + (double)fabsf(w.y);
+        b = (double)fabsf(d.x) + (double)fabsf(d.y);
+    }
+    a = block_sum_f64(a, red);
+    b = block_sum_f64(b, red);
+    if (threadIdx.x == 0) {
+        c.part_a[blockIdx.x] = a;
+        c.part_b[blockIdx.x] = b;
+    }
+}
+
+// g and the BB dot-product partials sum |dx|^2, sum |dg|^2 (nesterov.rs:87-93)
+__device__ __forceinline__ void bb_terms(const CellBufs &c, int64_t i, float2 g, double &a,
+                                         double &b) {
+    const float2 p = c.pos[i], pp = c.prev_pos[i], pg = c.prev_g[i];
+    const double dx = (double)(p.x - pp.x), dy = (double)(p.y - pp.y);
+    const double gx = (double)(g.x - pg.x), gy = (double)(g.y - pg.y);
+    a = dx * dx + dy * dy;
+    b = gx * gx + gy * gy;
+}
+
+__global__ void __launch_bounds__(kBlock) k_combine(CellBufs c, const DevState *__restrict__ st) {
+    __shared__ double red[kBlock];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0, b = 0.0;
+    if (i < c.C) {
+        const float2 g = combine(c, i, c.gwl[i], c.gd[i], (float)st->density_weight);
+        c.g[i] = g;
+        bb_terms(c, i, g, a, b);
+    }
+    a = block_sum_f64(a, red);
+    b = block_sum_f64(b, red);
+    if (threadIdx.x == 0) {
+        c.part_a[blockIdx.x] = a;
+        c.part_b[blockIdx.x] = b;
+    }
+}
+
+template <bool STORE_PARTS>
+__global__ void __launch_bounds__(kBlock) k_cell_fused(CellBufs c, const DevState *__restrict__ st) {
+    __shared__ double red[kBlock];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0, b = 0.0;
+    if (i < c.C) {
+        const float2 gwl = pin2cell(c, i);
+        const float2 gd = efield_cell(c, i, 1.f / c.bw, 1.f / c.bh);
+        const float2 g = combine(c, i, gwl, gd, (float)st->density_weight);
+        c.g[i] = g;
+        if (STORE_PARTS) {
+            c.gwl[i] = gwl;
+            c.gd[i] = gd;
+        }
+        bb_terms(c, i, g, a, b);
+    }
+    a = block_sum_f64(a, red);
+    b = block_sum_f64(b, red);
+    if (threadIdx.x == 0) {
+        c.part_a[blockIdx.x] = a;
+        c.part_b[blockIdx.x] = b;
+    }
+}
+
+// nesterov.rs:111-145
+__global__ void __launch_bounds__(kBlock) k_update(CellBufs c, const DevState *__restrict__ st) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.C) return;
+    const float alpha = (float)st->alpha, beta = (float)st->beta;
+    const float2 p = c.pos[i], pp = c.prev_pos[i], g = c.g[i], hi = c.hi[i];
+    const float xh = p.x + beta * (p.x - pp.x);
+    const float yh = p.y + beta * (p.y - pp.y);
+    float nx = xh - alpha * g.x, ny = yh - alpha * g.y;
+    if (nx < 0.f) nx = 0.f;      // f32::clamp(0, W - w - 1)
+    if (nx > hi.x) nx = hi.x;
+    if (ny < 0.f) ny = 0.f;
+    if (ny > hi.y) ny = hi.y;
+    c.prev_pos[i] = p;
+    c.pos_out[i] = make_float2(nx, ny);
+    c.prev_g[i] = g;
+}
+
+// Deterministic sum of `count` partials by one workgroup: fixed per-thread
+// strides, then a fixed tree.
+template <class T>
+__device__ T wg_sum(const T *part, int count, T *red) {
+    T v = 0;
+    for (int k = threadIdx.x; k < count; k += blockDim.x) v += part[k];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    const T r = red[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scalar(ScalarArgs a, int ops) {
+    __shared__ double redd[kBlock];
+    __shared__ int64_t redi[kBlock];
+    double hsum = 0, esum = 0, asum = 0, bsum = 0;
+    int64_t osum = 0;
+    if (ops & SC_HPWL) hsum = wg_sum(a.hpwl_part, a.n_hpwl, redd);
+    if (ops & SC_POT) {
+        osum = wg_sum(a.ovf_part, a.n_ovf, redi);
+        esum = wg_sum(a.energy_part, a.n_energy, redd);
+    }
+    if (ops & (SC_L1 | SC_NEST)) {
+        asum = wg_sum(a.part_a, a.n_cell, redd);
+        bsum = wg_sum(a.part_b, a.n_cell, redd);
+    }
+    if (threadIdx.x != 0) return;
+    DevState &s = *a.st;
+    if (ops & SC_HPWL) s.hpwl = hsum;
+    if (ops & SC_POT) {
+        s.overflow = (double)osum * a.inv_scale_over_area;
+        s.energy = esum;
+    }
+    if (ops & SC_L1) {                       // nesterov.rs:47-55
+        s.l1_wl = asum;
+        s.l1_d = bsum;
+        s.density_weight = s.density_weight * (asum / bsum);
+    }
+    if (ops & SC_NEST) {
+        // update_momentum (nesterov.rs:148-158), then the BB step (:98-100)
+        const double a_prev = s.alpha;
+        const double na = (1.0 + sqrt(4.0 * a_prev * a_prev + 1.0)) / 2.0;
+        s.beta = (a_prev - 1.0) / na;
+        s.alpha = na;
+        s.num = asum;
+        s.den = bsum;
+        if (s.iteration > 0 && bsum > 0.0) s.alpha = sqrt(asum / bsum);
+        s.dw_used = s.density_weight;
+    }
+    if (ops & SC_SCHED) {                    // scheduler.rs:68-130
+        s.iteration += 1;
+        if (!s.hpwl_has_initial) { s.hpwl_has_initial = 1; s.hpwl_initial = s.hpwl; }
+        s.hpwl_prev = s.hpwl_cur;
+        s.hpwl_cur = s.hpwl;
+        if (!s.ovf_has_initial) { s.ovf_has_initial = 1; s.ovf_initial = s.overflow; }
+        s.ovf_prev = s.ovf_cur;
+        s.ovf_cur = s.overflow;
+        const double ref = s.hpwl_has_initial ? s.hpwl_initial : 350000.0;
+        const double delta = s.hpwl_cur - s.hpwl_prev;
+        double power;
+        if (delta < 0.0) {
+            power = pow(0.9999, (double)(int32_t)s.iteration);
+            power = power > 0.98 ? power : 0.98;
+        } else {
+            power = pow(1.05, -delta / ref);
+            power = power < 0.95 ? 0.95 : (power > 1.05 ? 1.05 : power);
+        }
+        s.density_weight = s.density_weight * (1.05 * power);
+        double coef = pow(10.0, (s.ovf_cur - 0.1) * 20.0 / 9.0 - 1.0);
+        coef = coef < 0.05 ? 0.05 : (coef > 10.0 ? 10.0 : coef);
+        s.gamma = s.base_gamma * coef;
+        s.should_stop = s.hpwl_prev != 0.0 && s.hpwl_cur != 0.0 && s.hpwl_prev > s.hpwl_cur * 1.5;
+    }
+    const double chk = s.hpwl + s.overflow + s.energy + s.alpha + s.beta + s.density_weight + s.gamma;
+    s.nonfinite = isfinite(chk) ? 0 : 1;
+}
+
+__global__ void k_reset_state(DevState *st, double gamma0, double dw0) {
+    DevState s = {};
+    s.iteration = 0;
+    s.gamma = gamma0;
+    s.base_gamma = gamma0;
+    s.density_weight = dw0;
+    s.alpha = 1.0;
+    s.beta = 0.0;
+    *st = s;
+}
+
+// PlacementSolution::new (placement.rs:109-139) and Nesterov::new (nesterov.rs:15-24)
+__global__ void k_init_positions(float2 *pos, const float *w, const float *h, const float *x,
+                                 const float *y, int64_t N, int64_t C, float die_w, float die_h,
+                                 float2 *prev_pos, float2 *prev_g) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float two = 2.f;
+    if (i >= C) {
+        pos[i] = make_float2(x[i], y[i]);
+    } else {
+        pos[i] = make_float2(die_w / two - w[i] / two, die_h / two - h[i] / two);
+        prev_pos[i] = make_float2(x[i], y[i]);
+        prev_g[i] = make_float2(0.f, 0.f);
+    }
+}
+
+// epotential.rs:83-103 and the per-cell constants of nesterov.rs:75-79,131-132
+__global__ void k_setup_cells(const float *w, const float *h, const float *npins, int64_t N,
+                              int64_t C, float bw, float bh, float t, float die_w, float die_h,
+                              float2 *csize, float2 *coff, float *ratio, float *area,
+                              float *npins_out, float2 *hi) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float sqrt2 = 1.41421356237309504880f, half = 0.5f;
+    const float wi = w[i], hi_ = h[i];
+    const float cw = fmaxf(wi, bw * sqrt2), ch = fmaxf(hi_, bh * sqrt2);
+    csize[i] = make_float2(cw, ch);
+    coff[i] = make_float2((wi - cw) * half, (hi_ - ch) * half);
+    ratio[i] = i < C ? (wi * hi_) / (cw * ch) : t;
+    if (i < C) {
+        area[i] = wi * hi_;
+        npins_out[i] = npins[i];
+        hi[i] = make_float2(die_w - wi - 1.f, die_h - hi_ - 1.f);
+    }
+}
+
+inline unsigned grid_for(int64_t count) { return (unsigned)((count + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+int cell_partials(int64_t C) { return (int)grid_for(C > 0 ? C : 1); }
+
+#define ERP_LAUNCH_CELLS(kern, ...)                                                     \
+    do {                                                                                \
+        if (c.C <= 0) return ERP_OK;                                                    \
+        hipLaunchKernelGGL(kern, dim3(grid_for(c.C)), dim3(kBlock), 0, s, __VA_ARGS__); \
+        ERP_HIP_TRY(hipGetLastError());                                                 \
+        return ERP_OK;                                                                  \
+    } while (0)
+
+int launch_pin2cell(const CellBufs &c, hipStream_t s) { ERP_LAUNCH_CELLS(k_pin2cell, c); }
+int launch_efield(const CellBufs &c, hipStream_t s) { ERP_LAUNCH_CELLS(k_efield, c); }
+int launch_l1(const CellBufs &c, hipStream_t s) { ERP_LAUNCH_CELLS(k_l1, c); }
+int launch_combine(const CellBufs &c, const DevState *st, hipStream_t s) {
+    ERP_LAUNCH_CELLS(k_combine, c, st);
+}
+int launch_cell_fused(const CellBufs &c, const DevState *st, bool store_parts, hipStream_t s) {
+    if (store_parts) ERP_LAUNCH_CELLS(k_cell_fused<true>, c, st);
+    ERP_LAUNCH_CELLS(k_cell_fused<false>, c, st);
+}
+int launch_update(const CellBufs &c, const DevState *st, hipStream_t s) {
+    ERP_LAUNCH_CELLS(k_update, c, st);
+}
+
+int launch_scalar(const ScalarArgs &a, int ops, hipStream_t s) {
+    hipLaunchKernelGGL(k_scalar, dim3(1), dim3(kBlock), 0, s, a, ops);
+    ERP_HIP_TRY(hipGetLastError());
+    return ERP_OK;
+}
+
+int launch_reset_state(DevState *st, double gamma0, double dw0, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(1), 0, s, st, gamma0, dw0);
+    ERP_HIP_TRY(hipGetLastError());
+    return ERP_OK;
+}
+
+int launch_init_positions(float2 *pos, const float *w, const float *h, const float *x,
+                          const float *y, int64_t num_nodes, int64_t num_movable, float die_w,
+                          float die_h, float2 *prev_pos, float2 *prev_g, hipStream_t s) {
+    hipLaunchKernelGGL(k_init_positions, dim3(grid_for(num_nodes)), dim3(kBlock), 0, s, pos, w, h,
+                       x, y, num_nodes, num_movable, die_w, die_h, prev_pos, prev_g);
+    ERP_HIP_TRY(hipGetLastError());
+    return ERP_OK;
+}
+
+int launch_setup_cells(const float *w, const float *h, const float *npins, int64_t num_nodes,
+                       int64_t num_movable, float bw, float bh, float target_density,
+                       float die_w, float die_h, float2 *csize, float2 *coff, float *ratio,
+                       float *area, float *npins_out, float2 *hi, hipStream_t s) {
+    hipLaunchKernelGGL(k_setup_cells, dim3(grid_for(num_nodes)), dim3(kBlock), 0, s, w, h, npins,
+                       num_nodes, num_movable, bw, bh, target_density, die_w, die_h, csize, coff,
+                       ratio, area, npins_out, hi);
+    ERP_HIP_TRY(hipGetLastError());
+    return ERP_OK;
+}
+
+}  // namespace erp

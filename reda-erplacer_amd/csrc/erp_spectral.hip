@@ -1,0 +1,528 @@
+// Spectral Poisson solve, FFT form (FP32), for gfx950.
+//
+// The reference computes a 2-D DCT-II (dct2d.rs:256-279) and three inverse
+// transforms (idct2.rs:144-166, idct_idxst.rs:137-161, idxst_idct.rs:156-180),
+// each as a whole-map pre-process, a 2-D real FFT and a post-process.  Here every
+// 2-D transform is separable: a row pass and a column pass, each a batch of 1-D
+// transforms done in LDS by a workgroup.  A 1-D DCT-II of length L is Makhoul's
+// reordering v[p]=x[2p], v[L-1-p]=x[2p+1], one complex FFT, and a twiddle
+// e^{-i pi k/2L}; two real sequences share one complex FFT (real and imaginary
+// parts).  The inverse (2 sum_k eps_k Y_k cos(pi(2i+1)k/2L), eps_0=1/2) runs the
+// same steps backwards; the sine variant (2 sum_k Y_k sin(pi(2i+1)k/2L)) is the
+// cosine one on the reversed spectrum times (-1)^i.
+//
+// The fused potential pipeline is three passes over the map:
+//   pass 1 (rows):    fixed-point accumulators -> density map, overflow partials,
+//                     row DCT-II                                  (epotential.rs:271-297)
+//   pass 2 (columns): column DCT-II, the three spectral products, three column
+//                     inverses                                    (epotential.rs:299-331)
+//   pass 3 (rows):    three row inverses -> field_x, field_y, potential; energy
+//                     partials                                    (epotential.rs:310-346)
+#include <map>
+#include <mutex>
+
+#include "erp_internal.h"
+#include "erp_timer.h"
+
+namespace erp {
+namespace {
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ int brev(int v, int logL) {
+    return (int)(__brev((unsigned)v) >> (32 - logL));
+}
+// Makhoul order: element c of x sits at v[perm(c)]; x[i] = v[perm(i)] inverts it.
+__device__ __forceinline__ int perm(int c, int L) { return (c & 1) ? (L - 1 - (c >> 1)) : (c >> 1); }
+
+// In-place radix-2 decimation-in-time FFT of F sequences of length 2^logL held
+// in LDS at buf[f << logL], input already in bit-reversed order.  tw holds
+// e^{-2 pi i k / L}, k < L/2 (conjugated for the inverse).  Unnormalised.
+template <bool INV>
+__device__ void fft_stages(float2 *buf, int F, int logL, const float2 *tw) {
+    const int halfL = 1 << (logL - 1);
+    const int nb = F * halfL;
+    for (int s = 0; s < logL; ++s) {
+        __syncthreads();
+        const int half = 1 << s;
+        const int tsh = logL - 1 - s;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            const int f = b >> (logL - 1);
+            const int r = b & (halfL - 1);
+            const int k = r & (half - 1);
+            const int i0 = (f << logL) + ((r >> s) << (s + 1)) + k;
+            const int i1 = i0 + half;
+            float2 w = tw[k << tsh];
+            if (INV) w.y = -w.y;
+            const float2 u = buf[i0];
+            const float2 v = cmulf(buf[i1], w);
+            buf[i0] = make_float2(u.x + v.x, u.y + v.y);
+            buf[i1] = make_float2(u.x - v.x, u.y - v.y);
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void load_tw(float2 *dst, const float2 *src, int count) {
+    for (int k = threadIdx.x; k < count; k += blockDim.x) dst[k] = src[k];
+}
+
+// DCT-II of the two real sequences packed in Z (FFT output), at frequency k:
+// returns (y0[k], y1[k]) = (Re(w_k V0[k]), Re(w_k V1[k])) with the 1/2 of the
+// unpacking left to the caller's scale.
+__device__ __forceinline__ float2 dct_unpack(const float2 *Z, int k, int L, float2 w) {
+    const float2 zk = Z[k];
+    const float2 zr = Z[(L - k) & (L - 1)];
+    const float v0x = zk.x + zr.x, v0y = zk.y - zr.y;    // 2 V0
+    const float v1x = zk.y + zr.y, v1y = zr.x - zk.x;    // 2 V1
+    return make_float2(w.x * v0x - w.y * v0y, w.x * v1x - w.y * v1y);
+}
+
+// Inverse pre-twiddle of one real spectrum at k: e^{+i pi k/2L}(Yk - i Yr).
+__device__ __forceinline__ float2 inv_twiddle(float yk, float yr, float2 w) {
+    return make_float2(w.x * yk - w.y * yr, -w.x * yr - w.y * yk);
+}
+// Two real spectra packed for one complex inverse FFT: V0 + i V1.
+__device__ __forceinline__ float2 pack2(float2 v0, float2 v1) {
+    return make_float2(v0.x - v1.y, v0.y + v1.x);
+}
+// (Yk, Yr) for frequency k of an IDCT or (reversed spectrum) IDXST input.
+template <int KIND>
+__device__ __forceinline__ float2 spec_pair(const float *Y, int k, int L, int stride) {
+    if (KIND == INV_IDCT) {
+        return make_float2(Y[(size_t)k * stride], k ? Y[(size_t)(L - k) * stride] : 0.f);
+    } else {
+        return make_float2(k ? Y[(size_t)(L - k) * stride] : 0.f, k ? Y[(size_t)k * stride] : 0.f);
+    }
+}
+
+// ---------------------------------------------------------------- row DCT-II
+// R = 2F rows per block.  SRC_FIXED: rows come from the fixed-point accumulators
+// (pass 1); otherwise from a float map.  Output T = unscaled row DCT sums.
+template <bool SRC_FIXED>
+__global__ void __launch_bounds__(kBlock) k_row_dct(
+    const float *__restrict__ x, int64_t *__restrict__ q_mov, const int64_t *__restrict__ q_fix,
+    float *__restrict__ rho_out, int64_t *__restrict__ ovf_part, float *__restrict__ T, int n,
+    int logN, int F, const float2 *__restrict__ fft_tw, const float2 *__restrict__ dct_tw,
+    double inv_scale, int64_t coef_q, float inv_bin_area, bool clear_q) {
+    extern __shared__ float2 lds[];
+    float2 *tw = lds;
+    float2 *buf = lds + (n >> 1);
+    load_tw(tw, fft_tw, n >> 1);
+    const int row_base = blockIdx.x * 2 * F;
+    int64_t ovf = 0;
+    for (int idx = threadIdx.x; idx < (F << logN); idx += blockDim.x) {
+        const int f = idx >> logN, c = idx & (n - 1);
+        const size_t o0 = (size_t)(row_base + 2 * f) * n + c, o1 = o0 + n;
+        float a, b;
+        if (SRC_FIXED) {
+            const int64_t t0 = q_mov[o0] + q_fix[o0];
+            const int64_t t1 = q_mov[o1] + q_fix[o1];
+            if (clear_q) {
+                q_mov[o0] = 0;
+                q_mov[o1] = 0;
+            }
+            ovf += (t0 > coef_q ? t0 - coef_q : 0) + (t1 > coef_q ? t1 - coef_q : 0);
+            a = (float)((double)t0 * inv_scale) * inv_bin_area;
+            b = (float)((double)t1 * inv_scale) * inv_bin_area;
+            rho_out[o0] = a;
+            rho_out[o1] = b;
+        } else {
+            a = x[o0];
+            b = x[o1];
+        }
+        buf[(f << logN) + brev(perm(c, n), logN)] = make_float2(a, b);
+    }
+    if (SRC_FIXED) {
+        __shared__ int64_t red[kBlock];
+        red[threadIdx.x] = ovf;
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) ovf_part[blockIdx.x] = red[0];
+    }
+    fft_stages<false>(buf, F, logN, tw);
+    for (int idx = threadIdx.x; idx < (F << logN); idx += blockDim.x) {
+        const int f = idx >> logN, k = idx & (n - 1);
+        const float2 y = dct_unpack(buf + (f << logN), k, n, dct_tw[k]);
+        const size_t o0 = (size_t)(row_base + 2 * f) * n + k;
+        T[o0] = 0.5f * y.x;
+        T[o0 + n] = 0.5f * y.y;
+    }
+}
+
+// -------------------------------------------------------------- column DCT-II
+// S = 2G columns per block starting at column blockIdx -> strip (XCD-aware).
+// out[u][v] = scale * sum_i T[i][v] cos(pi(2i+1)u/2M)
+__device__ __forceinline__ int strip_of_block(int b, int nblocks) {
+    // Consecutive workgroups go round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of strips so neighbouring strips share its L2.
+    const int per = nblocks >> 3;
+    return (nblocks & 7) ? b : (b & 7) * per + (b >> 3);
+}
+
+__global__ void __launch_bounds__(kBlock) k_col_dct(const float *__restrict__ T,
+                                                    float *__restrict__ out, int m, int logM,
+                                                    int n, int G,
+                                                    const float2 *__restrict__ fft_tw,
+                                                    const float2 *__restrict__ dct_tw,
+                                                    float scale) {
+    extern __shared__ float2 lds[];
+    float2 *tw = lds;
+    float2 *buf = lds + (m >> 1);
+    load_tw(tw, fft_tw, m >> 1);
+    const int S = 2 * G;
+    const int v0 = strip_of_block(blockIdx.x, gridDim.x) * S;
+    for (int idx = threadIdx.x; idx < m * S; idx += blockDim.x) {
+        const int i = idx / S, s = idx - i * S;
+        const float val = T[(size_t)i * n + v0 + s];
+        float *dst = reinterpret_cast<float *>(&buf[((s >> 1) << logM) + brev(perm(i, m), logM)]);
+        dst[s & 1] = val;
+    }
+    fft_stages<false>(buf, G, logM, tw);
+    for (int idx = threadIdx.x; idx < m * G; idx += blockDim.x) {
+        const int u = idx / G, g = idx - u * G;
+        const float2 y = dct_unpack(buf + (g << logM), u, m, dct_tw[u]);
+        out[(size_t)u * n + v0 + 2 * g] = 0.5f * scale * y.x;
+        out[(size_t)u * n + v0 + 2 * g + 1] = 0.5f * scale * y.y;
+    }
+}
+
+// ---------------------------------------------------------- column inverse
+// out[i][v] = 2 sum_u eps_u X[u][v] cos(...)   (KIND = INV_IDCT)
+//           = 2 sum_u X[u][v] sin(...)         (KIND = INV_IDXST)
+template <int KIND>
+__global__ void __launch_bounds__(kBlock) k_col_inv(const float *__restrict__ X,
+                                                    float *__restrict__ out, int m, int logM,
+                                                    int n, int G,
+                                                    const float2 *__restrict__ fft_tw,
+                                                    const float2 *__restrict__ dct_tw) {
+    extern __shared__ float2 lds[];
+    float2 *tw = lds;
+    float2 *buf = lds + (m >> 1);
+    load_tw(tw, fft_tw, m >> 1);
+    const int S = 2 * G;
+    const int v0 = strip_of_block(blockIdx.x, gridDim.x) * S;
+    for (int idx = threadIdx.x; idx < m * G; idx += blockDim.x) {
+        const int k = idx / G, g = idx - k * G;
+        const float2 w = dct_tw[k];
+        const float2 p0 = spec_pair<KIND>(X + v0 + 2 * g, k, m, n);
+        const float2 p1 = spec_pair<KIND>(X + v0 + 2 * g + 1, k, m, n);
+        buf[(g << logM) + brev(k, logM)] = pack2(inv_twiddle(p0.x, p0.y, w), inv_twiddle(p1.x, p1.y, w));
+    }
+    fft_stages<true>(buf, G, logM, tw);
+    for (int idx = threadIdx.x; idx < m * S; idx += blockDim.x) {
+        const int i = idx / S, s = idx - i * S;
+        const float2 v = buf[((s >> 1) << logM) + perm(i, m)];
+        float r = (s & 1) ? v.y : v.x;
+        if (KIND == INV_IDXST && (i & 1)) r = -r;
+        out[(size_t)i * n + v0 + s] = r;
+    }
+}
+
+// ------------------------------------------------------------- row inverse
+// R = 2F rows per block; along y.
+template <int KIND>
+__global__ void __launch_bounds__(kBlock) k_row_inv(const float *__restrict__ X,
+                                                    float *__restrict__ out, int n, int logN,
+                                                    int F, const float2 *__restrict__ fft_tw,
+                                                    const float2 *__restrict__ dct_tw) {
+    extern __shared__ float2 lds[];
+    float2 *tw = lds;
+    float2 *buf = lds + (n >> 1);
+    load_tw(tw, fft_tw, n >> 1);
+    const int row_base = blockIdx.x * 2 * F;
+    for (int idx = threadIdx.x; idx < (F << logN); idx += blockDim.x) {
+        const int f = idx >> logN, k = idx & (n - 1);
+        const float *r0 = X + (size_t)(row_base + 2 * f) * n;
+        const float2 w = dct_tw[k];
+        const float2 p0 = spec_pair<KIND>(r0, k, n, 1);
+        const float2 p1 = spec_pair<KIND>(r0 + n, k, n, 1);
+        buf[(f << logN) + brev(k, logN)] = pack2(inv_twiddle(p0.x, p0.y, w), inv_twiddle(p1.x, p1.y, w));
+    }
+    fft_stages<true>(buf, F, logN, tw);
+    for (int idx = threadIdx.x; idx < (F << logN); idx += blockDim.x) {
+        const int f = idx >> logN, j = idx & (n - 1);
+        const float2 v = buf[(f << logN) + perm(j, n)];
+        const float sg = (KIND == INV_IDXST && (j & 1)) ? -1.f : 1.f;
+        const size_t o0 = (size_t)(row_base + 2 * f) * n + j;
+        out[o0] = sg * v.x;
+        out[o0 + n] = sg * v.y;
+    }
+}
+
+// ------------------------------------------------- fused column pass (pass 2)
+// Column DCT-II of the row-transformed map, a_uv = 4/(MN) * sums; the three
+// spectral products (frequencies.rs:4-72 evaluated in f32 on the fly, in the
+// reference's operation order); three column inverses:
+//   P1 = IDXST_x(a Wx), P2 = IDCT_x(a Wy), P3 = IDCT_x(a inv).
+__global__ void __launch_bounds__(kBlock) k_col_spectral(
+    const float *__restrict__ T, float *__restrict__ P1, float *__restrict__ P2,
+    float *__restrict__ P3, float *__restrict__ auv_out, int m, int logM, int n, int G,
+    const float2 *__restrict__ fft_tw, const float2 *__restrict__ dct_tw, float scale,
+    float two_pi_over_m, float two_pi_over_n, float wv_div) {
+    extern __shared__ float2 lds[];
+    const int S = 2 * G;
+    float2 *tw = lds;
+    float2 *buf = lds + (m >> 1);                          // 3G complex arrays of m
+    float *acol = reinterpret_cast<float *>(buf + 3 * G * m);  // S real arrays of m
+    load_tw(tw, fft_tw, m >> 1);
+    const int v0 = strip_of_block(blockIdx.x, gridDim.x) * S;
+    for (int idx = threadIdx.x; idx < m * S; idx += blockDim.x) {
+        const int i = idx / S, s = idx - i * S;
+        const float val = T[(size_t)i * n + v0 + s];
+        float *dst = reinterpret_cast<float *>(&buf[((s >> 1) << logM) + brev(perm(i, m), logM)]);
+        dst[s & 1] = val;
+    }
+    fft_stages<false>(buf, G, logM, tw);
+    for (int idx = threadIdx.x; idx < m * G; idx += blockDim.x) {
+        const int u = idx / G, g = idx - u * G;
+        const float2 y = dct_unpack(buf + (g << logM), u, m, dct_tw[u]);
+        const float a0 = 0.5f * scale * y.x, a1 = 0.5f * scale * y.y;
+        acol[(2 * g) * m + u] = a0;
+        acol[(2 * g + 1) * m + u] = a1;
+        if (auv_out) {
+            auv_out[(size_t)u * n + v0 + 2 * g] = a0;
+            auv_out[(size_t)u * n + v0 + 2 * g + 1] = a1;
+        }
+    }
+    __syncthreads();
+    // spectral products and inverse pre-twiddles, all three kinds
+    for (int idx = threadIdx.x; idx < m * G; idx += blockDim.x) {
+        const int k = idx / G, g = idx - k * G;
+        const int kr = (m - k) & (m - 1);
+        const float2 w = dct_tw[k];
+        float2 vx[2], vy[2], vp[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int s = 2 * g + h;
+            const int v = v0 + s;
+            const float wv = (float)v * two_pi_over_n * wv_div;
+            // operator values at rows k and m-k (frequencies.rs:22-57)
+            float wu_k = (float)k * two_pi_over_m, wu_r = (float)kr * two_pi_over_m;
+            float d_k = wu_k * wu_k + wv * wv, d_r = wu_r * wu_r + wv * wv;
+            float inv_k = (k == 0 && v == 0) ? 0.f : 1.f / d_k;
+            float inv_r = (kr == 0 && v == 0) ? 0.f : 1.f / d_r;
+            const float ak = acol[s * m + k];
+            const float ar = k ? acol[s * m + kr] : 0.f;
+            const float wxk = wu_k * inv_k * 0.5f, wxr = wu_r * inv_r * 0.5f;
+            const float wyk = wv * inv_k * 0.5f, wyr = wv * inv_r * 0.5f;
+            // IDXST along x of a*Wx: reversed spectrum (k -> m-k)
+            const float xk = k ? ar * wxr : 0.f, xr = k ? ak * wxk : 0.f;
+            vx[h] = inv_twiddle(xk, xr, w);
+            vy[h] = inv_twiddle(ak * wyk, ar * wyr, w);
+            vp[h] = inv_twiddle(ak * inv_k, ar * inv_r, w);
+        }
+        const int pos = brev(k, logM);
+        buf[((0 * G + g) << logM) + pos] = pack2(vx[0], vx[1]);
+        buf[((1 * G + g) << logM) + pos] = pack2(vy[0], vy[1]);
+        buf[((2 * G + g) << logM) + pos] = pack2(vp[0], vp[1]);
+    }
+    fft_stages<true>(buf, 3 * G, logM, tw);
+    for (int idx = threadIdx.x; idx < 3 * m * S; idx += blockDim.x) {
+        const int kind = idx / (m * S);
+        const int rem = idx - kind * m * S;
+        const int i = rem / S, s = rem - i * S;
+        const float2 v = buf[((kind * G + (s >> 1)) << logM) + perm(i, m)];
+        float r = (s & 1) ? v.y : v.x;
+        if (kind == 0 && (i & 1)) r = -r;
+        float *dst = kind == 0 ? P1 : (kind == 1 ? P2 : P3);
+        dst[(size_t)i * n + v0 + s] = r;
+    }
+}
+
+// ----------------------------------------------------- fused row pass (pass 3)
+// field_x = IDCT_y(P1), field_y = IDXST_y(P2), potential = IDCT_y(P3);
+// energy partial = sum potential * density (epotential.rs:344-346).
+__global__ void __launch_bounds__(kBlock) k_row_final(
+    const float *__restrict__ P1, const float *__restrict__ P2, const float *__restrict__ P3,
+    const float *__restrict__ rho, float *__restrict__ fx, float *__restrict__ fy,
+    float *__restrict__ pot, double *__restrict__ energy_part, int n, int logN, int F,
+    const float2 *__restrict__ fft_tw, const float2 *__restrict__ dct_tw) {
+    extern __shared__ float2 lds[];
+    float2 *tw = lds;
+    float2 *buf = lds + (n >> 1);   // 3F arrays
+    load_tw(tw, fft_tw, n >> 1);
+    const int row_base = blockIdx.x * 2 * F;
+    for (int idx = threadIdx.x; idx < (F << logN); idx += blockDim.x) {
+        const int f = idx >> logN, k = idx & (n - 1);
+        const size_t r0 = (size_t)(row_base + 2 * f) * n;
+        const float2 w = dct_tw[k];
+        const int pos = (f << logN) + brev(k, logN);
+        float2 a = spec_pair<INV_IDCT>(P1 + r0, k, n, 1), b = spec_pair<INV_IDCT>(P1 + r0 + n, k, n, 1);
+        buf[pos] = pack2(inv_twiddle(a.x, a.y, w), inv_twiddle(b.x, b.y, w));
+        a = spec_pair<INV_IDXST>(P2 + r0, k, n, 1);
+        b = spec_pair<INV_IDXST>(P2 + r0 + n, k, n, 1);
+        buf[pos + (F << logN)] = pack2(inv_twiddle(a.x, a.y, w), inv_twiddle(b.x, b.y, w));
+        a = spec_pair<INV_IDCT>(P3 + r0, k, n, 1);
+        b = spec_pair<INV_IDCT>(P3 + r0 + n, k, n, 1);
+        buf[pos + (2 * F << logN)] = pack2(inv_twiddle(a.x, a.y, w), inv_twiddle(b.x, b.y, w));
+    }
+    fft_stages<true>(buf, 3 * F, logN, tw);
+    double e = 0.0;
+    for (int idx = threadIdx.x; idx < (F << logN); idx += blockDim.x) {
+        const int f = idx >> logN, j = idx & (n - 1);
+        const int pos = (f << logN) + perm(j, n);
+        const float2 vx = buf[pos], vy = buf[pos + (F << logN)], vp = buf[pos + (2 * F << logN)];
+        const float sg = (j & 1) ? -1.f : 1.f;
+        const size_t o0 = (size_t)(row_base + 2 * f) * n + j;
+        fx[o0] = vx.x;
+        fx[o0 + n] = vx.y;
+        fy[o0] = sg * vy.x;
+        fy[o0 + n] = sg * vy.y;
+        pot[o0] = vp.x;
+        pot[o0 + n] = vp.y;
+        e += (double)(vp.x * rho[o0]) + (double)(vp.y * rho[o0 + n]);
+    }
+    __shared__ double red[kBlock];
+    red[threadIdx.x] = e;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) energy_part[blockIdx.x] = red[0];
+}
+
+__global__ void k_make_tables(float2 *fft_tw, float2 *dct_tw, int L) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < L / 2) {
+        double s, c;
+        sincospi(-2.0 * (double)k / (double)L, &s, &c);
+        fft_tw[k] = make_float2((float)c, (float)s);
+    }
+    if (k < L) {
+        double s, c;
+        sincospi(-(double)k / (2.0 * (double)L), &s, &c);
+        dct_tw[k] = make_float2((float)c, (float)s);
+    }
+}
+
+// rows per block of the row passes, columns per block of the column pass
+constexpr int kRowPairs = 1;   // F: one packed pair of rows per workgroup
+inline int col_pairs(int m) { return m >= 2048 ? 1 : 2; }
+
+size_t row_lds(int n, int arrays) { return sizeof(float2) * ((size_t)n / 2 + (size_t)arrays * n); }
+
+// Raise a kernel's dynamic-LDS limit once per (device, kernel) to the largest size
+// requested so far, so that captured launches never need the attribute call.
+int set_lds_limit(const void *fn, size_t bytes) {
+    if (bytes <= 48 * 1024) return ERP_OK;
+    static std::mutex mu;
+    static std::map<std::pair<int, const void *>, size_t> done;
+    int dev = 0;
+    ERP_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    size_t &cur = done[{dev, fn}];
+    if (bytes <= cur) return ERP_OK;
+    ERP_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    cur = bytes;
+    return ERP_OK;
+}
+
+}  // namespace
+
+int make_spec_tables(SpecTables &t, int L, hipStream_t s) {
+    t.L = L;
+    int rc = t.fft_tw.alloc(sizeof(float2) * (size_t)(L / 2));
+    if (rc) return rc;
+    if ((rc = t.dct_tw.alloc(sizeof(float2) * (size_t)L))) return rc;
+    hipLaunchKernelGGL(k_make_tables, dim3((L + 255) / 256), dim3(256), 0, s,
+                       t.fft_tw.as<float2>(), t.dct_tw.as<float2>(), L);
+    ERP_HIP_TRY(hipGetLastError());
+    return ERP_OK;
+}
+
+int potential_partials(int m) { return m / (2 * kRowPairs); }
+
+int launch_dct2(const float *x, float *out, float *scratch, int m, int n, const SpecTables &tm,
+                const SpecTables &tn, hipStream_t s) {
+    const int F = kRowPairs, logN = ilog2(n), logM = ilog2(m), G = col_pairs(m);
+    size_t lr = row_lds(n, F);
+    int rc = set_lds_limit((const void *)k_row_dct<false>, lr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_row_dct<false>, dim3(m / (2 * F)), dim3(kBlock), lr, s, x, nullptr,
+                       nullptr, nullptr, nullptr, scratch, n, logN, F, tn.fft_tw.as<float2>(),
+                       tn.dct_tw.as<float2>(), 0.0, (int64_t)0, 0.f, false);
+    ERP_HIP_TRY(hipGetLastError());
+    size_t lc = row_lds(m, G);
+    if ((rc = set_lds_limit((const void *)k_col_dct, lc))) return rc;
+    hipLaunchKernelGGL(k_col_dct, dim3(n / (2 * G)), dim3(kBlock), lc, s, scratch, out, m, logM,
+                       n, G, tm.fft_tw.as<float2>(), tm.dct_tw.as<float2>(),
+                       (float)(4.0 / ((double)m * (double)n)));
+    ERP_HIP_TRY(hipGetLastError());
+    return ERP_OK;
+}
+
+int launch_inverse2(const float *x, float *out, float *scratch, int m, int n, InvKind kind_x,
+                    InvKind kind_y, const SpecTables &tm, const SpecTables &tn, hipStream_t s) {
+    const int F = kRowPairs, logN = ilog2(n), logM = ilog2(m), G = col_pairs(m);
+    size_t lc = row_lds(m, G);
+    const void *colk = kind_x == INV_IDCT ? (const void *)k_col_inv<INV_IDCT>
+                                          : (const void *)k_col_inv<INV_IDXST>;
+    int rc = set_lds_limit(colk, lc);
+    if (rc) return rc;
+    if (kind_x == INV_IDCT)
+        hipLaunchKernelGGL(k_col_inv<INV_IDCT>, dim3(n / (2 * G)), dim3(kBlock), lc, s, x, scratch,
+                           m, logM, n, G, tm.fft_tw.as<float2>(), tm.dct_tw.as<float2>());
+    else
+        hipLaunchKernelGGL(k_col_inv<INV_IDXST>, dim3(n / (2 * G)), dim3(kBlock), lc, s, x,
+                           scratch, m, logM, n, G, tm.fft_tw.as<float2>(), tm.dct_tw.as<float2>());
+    ERP_HIP_TRY(hipGetLastError());
+    size_t lr = row_lds(n, F);
+    const void *rowk = kind_y == INV_IDCT ? (const void *)k_row_inv<INV_IDCT>
+                                          : (const void *)k_row_inv<INV_IDXST>;
+    if ((rc = set_lds_limit(rowk, lr))) return rc;
+    if (kind_y == INV_IDCT)
+        hipLaunchKernelGGL(k_row_inv<INV_IDCT>, dim3(m / (2 * F)), dim3(kBlock), lr, s, scratch,
+                           out, n, logN, F, tn.fft_tw.as<float2>(), tn.dct_tw.as<float2>());
+    else
+        hipLaunchKernelGGL(k_row_inv<INV_IDXST>, dim3(m / (2 * F)), dim3(kBlock), lr, s, scratch,
+                           out, n, logN, F, tn.fft_tw.as<float2>(), tn.dct_tw.as<float2>());
+    ERP_HIP_TRY(hipGetLastError());
+    return ERP_OK;
+}
+
+int launch_potential_fft(const PotentialBufs &b, const SpecTables &tm, const SpecTables &tn,
+                         hipStream_t s, KernelTimer *t) {
+    const int m = b.m, n = b.n, F = kRowPairs, logN = ilog2(n), logM = ilog2(m), G = col_pairs(m);
+    // pass 1
+    size_t l1 = row_lds(n, F);
+    int rc = set_lds_limit((const void *)k_row_dct<true>, l1);
+    if (rc) return rc;
+    if (t) t->begin("pass1_row_dct");
+    hipLaunchKernelGGL(k_row_dct<true>, dim3(m / (2 * F)), dim3(kBlock), l1, s, nullptr, b.q_mov,
+                       b.q_fix, b.rho, b.ovf_part, b.rowt, n, logN, F, tn.fft_tw.as<float2>(),
+                       tn.dct_tw.as<float2>(), b.inv_scale, b.coef_q, b.inv_bin_area,
+                       !b.keep_q);
+    if (t) t->end();
+    ERP_HIP_TRY(hipGetLastError());
+    // pass 2 (operator constants exactly as frequencies.rs computes them in T=f32)
+    const float two_pi_over_m = (float)(2.0 * M_PI) / (float)m;
+    const float two_pi_over_n = (float)(2.0 * M_PI) / (float)n;
+    const float wv_div = b.bw / b.bh;
+    size_t l2 = sizeof(float2) * ((size_t)m / 2 + (size_t)3 * G * m) + sizeof(float) * (size_t)2 * G * m;
+    if ((rc = set_lds_limit((const void *)k_col_spectral, l2))) return rc;
+    if (t) t->begin("pass2_col_spectral");
+    hipLaunchKernelGGL(k_col_spectral, dim3(n / (2 * G)), dim3(kBlock), l2, s, b.rowt, b.p1, b.p2,
+                       b.p3, b.auv, m, logM, n, G, tm.fft_tw.as<float2>(), tm.dct_tw.as<float2>(),
+                       (float)(4.0 / ((double)m * (double)n)), two_pi_over_m, two_pi_over_n,
+                       wv_div);
+    if (t) t->end();
+    ERP_HIP_TRY(hipGetLastError());
+    // pass 3
+    size_t l3 = row_lds(n, 3 * F);
+    if ((rc = set_lds_limit((const void *)k_row_final, l3))) return rc;
+    if (t) t->begin("pass3_row_final");
+    hipLaunchKernelGGL(k_row_final, dim3(m / (2 * F)), dim3(kBlock), l3, s, b.p1, b.p2, b.p3,
+                       b.rho, b.fx, b.fy, b.pot, b.energy_part, n, logN, F,
+                       tn.fft_tw.as<float2>(), tn.dct_tw.as<float2>());
+    if (t) t->end();
+    ERP_HIP_TRY(hipGetLastError());
+    return ERP_OK;
+}
+
+}  // namespace erp

@@ -1,0 +1,143 @@
+"""ctypes bindings to liberp_gpu.so (the C ABI declared in include/erp_gpu.h).
+
+The library is the product: every call here goes to the HIP kernels.  There is no
+CPU fallback; a missing library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "_lib", "liberp_gpu.so")
+HEADER = os.path.join(os.path.dirname(os.path.dirname(PKG_DIR)), "include", "erp_gpu.h")
+
+ERP_OK, ERP_ERR_INVALID, ERP_ERR_HIP, ERP_ERR_NO_DEVICE, ERP_ERR_UNSUPPORTED = 0, -1, -2, -3, -4
+SPECTRAL_FFT, SPECTRAL_MFMA_BF16X3 = 0, 1
+MAP_DENSITY, MAP_AUV, MAP_FIELD_X, MAP_FIELD_Y, MAP_POTENTIAL, MAP_INITIAL = range(6)
+GRAD_WIRELENGTH, GRAD_DENSITY, GRAD_COMBINED = range(3)
+
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+
+
+class ErpError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"erp error {code}: {msg}")
+        self.code = code
+
+
+class Design(C.Structure):
+    _fields_ = [
+        ("num_nodes", C.c_int64), ("num_movable", C.c_int64),
+        ("x", _fp), ("y", _fp), ("w", _fp), ("h", _fp), ("num_pins", _fp),
+        ("num_pins_total", C.c_int64),
+        ("pin_offset_x", _fp), ("pin_offset_y", _fp), ("pin_node", _ip),
+        ("num_nets", C.c_int64), ("n_2nets", C.c_int64),
+        ("net_start", _ip), ("net_pins", _ip),
+        ("node_pin_start", _ip), ("node_pins", _ip),
+        ("die_w", C.c_double), ("die_h", C.c_double),
+        ("movable_area", C.c_double), ("cell_utilization", C.c_double),
+    ]
+
+
+class Config(C.Structure):
+    _fields_ = [("num_bins_x", C.c_int32), ("num_bins_y", C.c_int32),
+                ("target_density", C.c_double), ("gamma_scale", C.c_double),
+                ("density_weight0", C.c_double), ("spectral_mode", C.c_int32),
+                ("device", C.c_int32)]
+
+
+class StepInfo(C.Structure):
+    _fields_ = [("hpwl", C.c_double), ("overflow", C.c_double), ("energy", C.c_double),
+                ("density_weight", C.c_double), ("gamma", C.c_double),
+                ("alpha", C.c_double), ("beta", C.c_double),
+                ("iteration", C.c_int64), ("should_stop", C.c_int32), ("nonfinite", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def declared_functions():
+    """Names of the functions include/erp_gpu.h declares."""
+    with open(HEADER) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(erp_[a-z0-9_]+)\s*\(", text)))
+
+
+def load(path: str | None = None):
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise ErpError(ERP_ERR_NO_DEVICE,
+                       f"{path} not built; run `make -C reda-erplacer_amd` (or __graft_entry__.build())")
+    lib = C.CDLL(path)
+    lib.erp_last_error.restype = C.c_char_p
+    lib.erp_config_default.restype = None
+    for name in declared_functions():
+        fn = getattr(lib, name)
+        if name not in ("erp_last_error", "erp_config_default"):
+            fn.restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != ERP_OK:
+        raise ErpError(rc, load().erp_last_error().decode(errors="replace"))
+    return rc
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = load().erp_device_count(C.byref(n))
+    return n.value if rc == ERP_OK else 0
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def design_struct(d):
+    """Design over a generator Design's arrays (keeps them referenced)."""
+    keep = []
+
+    def f(a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        keep.append(a)
+        return a.ctypes.data_as(_fp)
+
+    def i(a):
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        keep.append(a)
+        return a.ctypes.data_as(_ip)
+
+    s = Design(num_nodes=d.num_nodes, num_movable=d.num_movable,
+               x=f(d.x), y=f(d.y), w=f(d.w), h=f(d.h), num_pins=f(d.num_pins),
+               num_pins_total=d.num_pin_total,
+               pin_offset_x=f(d.pin_offset_x), pin_offset_y=f(d.pin_offset_y),
+               pin_node=i(d.pin_node), num_nets=d.num_nets, n_2nets=d.n_2nets,
+               net_start=i(d.net_start), net_pins=i(d.net_pins),
+               node_pin_start=i(d.node_pin_start), node_pins=i(d.node_pins),
+               die_w=d.die_w, die_h=d.die_h, movable_area=d.movable_area,
+               cell_utilization=d.cell_utilization)
+    s._keep = keep
+    return s
+
+
+def default_config(**kw) -> Config:
+    cfg = Config()
+    load().erp_config_default(C.byref(cfg))
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg

@@ -1,0 +1,186 @@
+"""Placement::step on the GPU against the CPU oracle (placement.rs:180-228).
+
+Step 0 is checked stage by stage (setup arrays, WA gradient and HPWL, density
+map bit for bit, overflow, field/potential maps, density gradient, the Nesterov
+update); later steps through the returned scalars; the graph-replayed loop
+against the per-phase loop bit for bit; a whole small run against the FP64 run.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+# Against the FP64 oracle, f32 cell coordinates (the reference binary's own
+# precision) bound overlap and pin-distance accuracy: ~grid * 6e-8 relative for
+# density-derived quantities, correlated when all cells sit at the die centre.
+TOL_MAP = 5e-5
+
+
+def _design(cells=4000, nets=4200, macros=8, grid=64, seed=5):
+    from erplacer.netlist import generate
+    d = generate(cells, nets, 32, macros, 0.10, 0.7, seed)
+    return d, grid
+
+
+@pytest.fixture(scope="module")
+def small():
+    return _design()
+
+
+def test_setup_matches_oracle(gpu_lib, small):
+    """PotentialComputation::new clamped sizes / offsets / ratios (epotential.rs:83-103)."""
+    from erplacer.placement import Placement
+    d, g = small
+    with Placement(d, g, target_density=1.0) as P:
+        s = P.setup()
+    o = O.Placer(d, g, target_density=1.0, sfx="_f32", fixed_point=True).setup()
+    for k in ("cw", "ch", "ox", "oy", "ratio"):
+        assert np.array_equal(s[k], o[k]), k
+    assert s["bw"] == o["bw"] and s["bh"] == o["bh"]
+
+
+def test_step0_stages(gpu_lib, small):
+    from erplacer import lib as L
+    from erplacer.placement import Placement
+    d, g = small
+    P = Placement(d, g, target_density=1.0)
+    P.set_keep_density(True)
+    ref = O.Placer(d, g, target_density=1.0)                    # FP64, faithful
+    x0, y0 = P.positions()
+    ox0, oy0 = ref.positions()
+    assert np.array_equal(x0.astype(np.float64), ox0) and np.array_equal(y0.astype(np.float64), oy0)
+
+    h = P.compute_hpwl()
+    energy, overflow = P.compute_potential()
+    r = ref.step()
+    assert math.isclose(h, r["hpwl"], rel_tol=1e-5)
+    assert abs(overflow - r["overflow"]) < 2e-5
+    assert math.isclose(energy, r["energy"], rel_tol=TOL_MAP)
+
+    gx, gy = P.cell_grads(L.GRAD_WIRELENGTH)
+    rx, ry = ref.cell_grads(O.GRAD_WL)
+    assert rel_l2(gx, rx) < 1e-5 and rel_l2(gy, ry) < 1e-5
+    for which_gpu, which_ref in ((L.MAP_DENSITY, O.MAP_DENSITY), (L.MAP_FIELD_X, O.MAP_FIELD_X),
+                                 (L.MAP_FIELD_Y, O.MAP_FIELD_Y), (L.MAP_POTENTIAL, O.MAP_POTENTIAL)):
+        assert rel_l2(P.map(which_gpu), ref.map(which_ref)) < TOL_MAP, which_gpu
+    gx, gy = P.cell_grads(L.GRAD_DENSITY)
+    rx, ry = ref.cell_grads(O.GRAD_DENSITY)
+    assert rel_l2(gx, rx) < TOL_MAP and rel_l2(gy, ry) < TOL_MAP
+
+    # density map bit-exact against the CPU fixed-point restatement
+    q_mov, q_fix, scale = P.density_fixed()
+    st = P.setup()
+    C = d.num_movable
+    q_ref = O.density_map_fx64(x0[:C], y0[:C], st["cw"][:C], st["ch"][:C], st["ox"][:C],
+                               st["oy"][:C], st["ratio"][:C], g, g, st["bw"], st["bh"], scale,
+                               sfx="_f32")
+    assert np.array_equal(q_mov, q_ref)
+    rho, ovf_q = O.fx_finalize(q_mov, q_fix, scale, 1.0 * float(st["bw"]) * float(st["bh"]))
+    gpu_rho = P.map(L.MAP_DENSITY).reshape(-1)
+    inv_area = np.float32(1.0) / (np.float32(st["bw"]) * np.float32(st["bh"]))
+    assert np.array_equal(gpu_rho, (rho * inv_area).astype(np.float32))
+    assert math.isclose(overflow, ovf_q / scale / d.movable_area, rel_tol=1e-12)
+
+    # Nesterov (iteration 0: golden-ratio step, beta = 0, L1-scaled weight) + scheduler
+    P.nesterov_step()
+    P.scheduler_update()
+    info = P.info()
+    assert info["iteration"] == 1 and info["beta"] == 0.0
+    assert math.isclose(info["alpha"], (1 + math.sqrt(5)) / 2, rel_tol=1e-12)
+    assert math.isclose(info["density_weight"], r["density_weight"], rel_tol=1e-4)
+    # gamma = g0 * 10^((ovf - 0.1) 20/9 - 1): d(gamma)/gamma = 5.1 d(ovf)
+    assert math.isclose(info["gamma"], r["gamma"], rel_tol=2e-4)
+    x1, y1 = P.positions()
+    rx1, ry1 = ref.positions()
+    assert np.max(np.abs(x1 - rx1)) < 0.05 and np.max(np.abs(y1 - ry1)) < 0.05
+
+
+def test_steps_track_oracle(gpu_lib, small):
+    """The first iterations track the FP64 run closely (trajectory diverges later)."""
+    from erplacer.placement import Placement
+    d, g = small
+    P = Placement(d, g, target_density=1.0)
+    ref = O.Placer(d, g, target_density=1.0)
+    for k in range(10):
+        a = P.step()
+        b = ref.step()
+        assert math.isclose(a["hpwl"], b["hpwl"], rel_tol=1e-3), k
+        assert abs(a["overflow"] - b["overflow"]) < 1e-3, k
+
+
+def test_graph_loop_matches_phase_loop(gpu_lib, small):
+    """erp_run (captured hipGraphs, fused per-cell kernel) gives bit-identical
+    positions to erp_step called repeatedly (separate phase kernels)."""
+    from erplacer.placement import Placement
+    d, g = small
+    A = Placement(d, g, target_density=1.0)
+    B = Placement(d, g, target_density=1.0)
+    for _ in range(25):
+        A.step()
+    B.run(25)
+    xa, ya = A.positions()
+    xb, yb = B.positions()
+    assert np.array_equal(xa, xb) and np.array_equal(ya, yb)
+    assert A.info()["hpwl"] == B.info()["hpwl"]
+
+
+def test_deterministic(gpu_lib, small):
+    from erplacer.placement import Placement
+    d, g = small
+    runs = []
+    for _ in range(2):
+        P = Placement(d, g, target_density=1.0)
+        P.run(40)
+        runs.append(P.positions())
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+
+
+def test_reset_restarts_trajectory(gpu_lib, small):
+    from erplacer.placement import Placement
+    d, g = small
+    P = Placement(d, g, target_density=1.0)
+    a = P.run(15)
+    P.reset()
+    b = P.run(15)
+    assert a["hpwl"] == b["hpwl"] and a["overflow"] == b["overflow"]
+
+
+def test_final_hpwl_dbu_exact(gpu_lib, small):
+    from erplacer.placement import Placement
+    d, g = small
+    P = Placement(d, g, target_density=1.0)
+    P.run(30)
+    x, y = P.positions()
+    px = np.rint((x[d.pin_node] + d.pin_offset_x).astype(np.float32)).astype(np.int64)
+    py = np.rint((y[d.pin_node] + d.pin_offset_y).astype(np.float32)).astype(np.int64)
+    tot = 0
+    for e in range(d.num_nets):
+        s, t = d.net_start[e], d.net_start[e + 1]
+        pins = d.net_pins[s:t]
+        tot += int(px[pins].max() - px[pins].min() + py[pins].max() - py[pins].min())
+    assert P.final_hpwl_dbu() == tot
+
+
+def test_small_workload_end_state(gpu_lib):
+    """Whole 'small' workload (10k cells, 128x128, 300 iterations): final HPWL
+    and overflow against the FP64 oracle run.  The reference's conflated
+    momentum makes trajectories chaotic, so this is an end-state check."""
+    from erplacer import workloads as W
+    from erplacer.placement import Placement
+    d, g, iters, td = W.make("small")
+    P = Placement(d, g, target_density=td)
+    a = P.run(iters)
+    ref = O.Placer(d, g, target_density=td)
+    for _ in range(iters):
+        b = ref.step()
+    rel = abs(a["hpwl"] / b["hpwl"] - 1)
+    print(f"final hpwl gpu {a['hpwl']:.6e} oracle {b['hpwl']:.6e} rel {rel:.4%}; "
+          f"overflow gpu {a['overflow']:.5f} oracle {b['overflow']:.5f}")
+    assert not a["nonfinite"]
+    assert rel < 0.02
+    assert abs(a["overflow"] - b["overflow"]) < 0.01
