@@ -28,37 +28,44 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "GP iters/s @1M cells & placements/min + p50/p99 latency @ HPWL within 0.5%"
 
 
+WA_BUCKETS = [("wa_deg4", 2, 4), ("wa_deg8", 4, 8), ("wa_deg16", 8, 16), ("wa_deg32", 16, 32),
+              ("wa_deg64", 32, 64), ("wa_deg128", 64, 128), ("wa_deg256", 128, 256),
+              ("wa_deg512", 256, 512), ("wa_large", 512, 1 << 30)]
+
+
 def kernel_bytes(name, d, grid, cells_pins):
-    """Algorithmic HBM bytes of one launch of `name` (each logical array read or
-    written once per launch)."""
+    """Algorithmic HBM bytes of one launch of `name`: each logical array the
+    kernel needs is read or written once (gathers count the gathered element)."""
     import numpy as np
-    C, E, P = d.num_movable, d.num_nets, d.num_pin_total
+    C = d.num_movable
     deg = np.diff(d.net_start)
     B = grid * grid
     n2 = int(d.n_2nets)
-    multi_small = deg[n2:][deg[n2:] <= 32]
-    multi_large = deg[n2:][deg[n2:] > 32]
+    multi = deg[n2:]
     table = {
-        # slot node (4) + pin offset (8) + instance position gather (8) + pin grad write (8)
+        # per pin: slot node (4) + pin offset (8) + instance position gather (8) + grad write (8)
         "wa_2pin": n2 * 2 * 28,
-        "wa_small": int(multi_small.sum()) * 28 + multi_small.size * 4,
-        "wa_large": int(multi_large.sum()) * 28 + multi_large.size * 4,
-        # cell position, clamped size, offset (8 each) + ratio (4); int64 map written once
-        "scatter": C * 28 + B * 8,
+        # permuted cell id (4) + position gather (8) + size/offset record (16) + ratio (4);
+        # int64 map written once
+        "scatter": C * 32 + B * 8,
+        "sort_keys": C * 20,
+        "sort_place": C * 48,
         # read q_mov, q_fix (16), clear q_mov (8), write density (4) and row transform (4)
         "pass1_row_dct": B * 32,
         # read row transform (4), write three column-inverse maps (12)
         "pass2_col_spectral": B * 16,
         # read three maps (12) + density (4), write field x/y and potential (12)
         "pass3_row_final": B * 28,
-        # per cell: CSR start (4), pin slots (4+8 per pin), position/offset/size/ratio (28),
-        # pins/area (8), previous position and gradient (16), gradient write (8),
-        # field maps read once (8 per bin)
+        # per cell: CSR start (4), position/offset/size/ratio (28), pins/area (8),
+        # previous position and gradient (16), gradient write (8); per pin: slot id (4) +
+        # pin gradient (8); field maps read once (8 per bin)
         "cell_fused": C * (4 + 28 + 8 + 16 + 8) + cells_pins * 12 + B * 8,
         # position, previous position, gradient, clamp bound read; three writes
         "update": C * 56,
-        "scalar": 0,
     }
+    for nm, lo, hi in WA_BUCKETS:
+        sel = multi[(multi > lo) & (multi <= hi)] if nm != "wa_deg4" else multi[multi <= hi]
+        table[nm] = int(sel.sum()) * 28 + int(sel.size) * 8
     return table.get(name, 0)
 
 
@@ -80,9 +87,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="large")
     ap.add_argument("--spectral", default="fft", choices=["fft", "mfma"])
-    ap.add_argument("--timing-steps", type=int, default=100,
-                    help="iterations re-run with per-kernel HIP events for the roofline")
-    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--timing-steps", type=int, default=0,
+                    help="iterations re-run with per-kernel HIP events for the roofline "
+                         "(default: the same number as --steps)")
+    ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true",
@@ -93,7 +101,6 @@ def main():
               "(jobs shard across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     from erplacer import lib as L
     from erplacer import workloads as W
     from erplacer.placement import Placement
@@ -123,16 +130,17 @@ def main():
     # per-kernel device time with HIP events over a second timed region
     P.reset()
     P.set_timing(True)
-    P.run(min(args.timing_steps, args.steps))
+    P.run(args.timing_steps or args.steps)
     P.set_timing(False)
     times = P.timing()
-    P.set_timing(False)
-    pins_cells = int(sum(int(d.node_pin_start[c + 1] - d.node_pin_start[c]) for c in [0]) * 0)
+    if not times:
+        raise RuntimeError("no kernel timings were recorded")
     pins_cells = int(d.node_pin_start[d.num_movable])
     per_iter = {}
     for k, ms in times.items():
         per_iter[k] = ms
-    dom = max((k for k in per_iter if k != "scalar"), key=lambda k: per_iter[k])
+    dom = max((k for k in per_iter if kernel_bytes(k, d, grid, pins_cells) > 0),
+              key=lambda k: per_iter[k])
     dom_ms = per_iter[dom]
     nbytes = kernel_bytes(dom, d, grid, pins_cells)
     achieved = nbytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0

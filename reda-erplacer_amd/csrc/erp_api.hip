@@ -21,7 +21,6 @@ int fail(int code, const std::string &msg) {
     return code;
 }
 
-constexpr int kSmallDegree = 32;     // nets up to this degree: one per thread
 constexpr int kGraphIters = 8;       // iterations per captured graph
 
 double fx_scale(double mass_bound) {
@@ -57,6 +56,8 @@ struct erp_ctx {
     DevBuf q_mov, q_fix, rho, rowt, p1, p2, p3, fx, fy, pot, auv;
     DevBuf hpwl_part, ovf_part, energy_part, part_a, part_b;
     DevBuf state, gamma_override, hpwl_dbu;
+    DevBuf keys, perm, perm_out, hist, cursor, rec, rratio;
+    SortBufs sb{};
     SpecTables tm, tn;
     MfmaPlan *mfma = nullptr;
     WaNets wa{};
@@ -133,10 +134,11 @@ int run_potential(erp_ctx *ctx) {
         if ((rc = launch_zero_i64(ctx->q_mov.as<int64_t>(), ctx->B, ctx->stream))) return rc;
         ctx->q_dirty = false;
     }
+    if ((rc = launch_sort(ctx->sb, ctx->stream, t))) return rc;
     if (t) t->begin("scatter");
-    rc = launch_scatter(ctx->pos.as<float2>(), ctx->csize.as<float2>(), ctx->coff.as<float2>(),
-                        ctx->ratio.as<float>(), ctx->C, ctx->m, ctx->n, ctx->bw, ctx->bh,
-                        ctx->fx_scale, ctx->q_mov.as<int64_t>(), ctx->stream);
+    rc = launch_scatter(ctx->rec.as<float4>(), ctx->rratio.as<float>(), ctx->C, ctx->m, ctx->n,
+                        ctx->bw, ctx->bh, ctx->fx_scale, ctx->perm_out.as<int32_t>(),
+                        ctx->perm.as<int32_t>(), ctx->q_mov.as<int64_t>(), ctx->stream);
     if (t) t->end();
     if (rc) return rc;
     PotentialBufs b = ctx->pb;
@@ -310,10 +312,13 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     std::stable_sort(multi.begin(), multi.end(), [&](int64_t a, int64_t b) {
         return (d->net_start[a + 1] - d->net_start[a]) < (d->net_start[b + 1] - d->net_start[b]);
     });
-    int64_t small_end = d->n_2nets;
+    int64_t bucket_count[kWaBuckets] = {0};
     for (int64_t e : multi) {
         order.push_back(e);
-        if (d->net_start[e + 1] - d->net_start[e] <= kSmallDegree) ++small_end;
+        const int64_t deg = d->net_start[e + 1] - d->net_start[e];
+        int b = 0;
+        while (deg > kWaDegree[b]) ++b;
+        ++bucket_count[b];
     }
     std::vector<int32_t> h_slot_node, h_net_start;
     std::vector<float2> h_slot_off;
@@ -413,10 +418,8 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     w.slot_off = ctx->slot_off.as<float2>();
     w.net_start = ctx->net_start.as<int32_t>();
     w.n2 = d->n_2nets;
-    w.small_begin = d->n_2nets;
-    w.small_end = small_end;
-    w.large_begin = small_end;
-    w.large_end = E;
+    w.bucket_begin[0] = d->n_2nets;
+    for (int b = 0; b < kWaBuckets; ++b) w.bucket_begin[b + 1] = w.bucket_begin[b] + bucket_count[b];
     w.slot_grad = ctx->slot_grad.as<float2>();
     ctx->n_hpwl_part = wa_partials(w);
     TRY(ctx->hpwl_part.alloc(sizeof(double) * (size_t)std::max(ctx->n_hpwl_part, 1)));
@@ -495,7 +498,35 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     sa.inv_scale_over_area = 1.0 / (ctx->fx_scale * ctx->movable_area);
 
     // ------------------------------------------------------------- device setup kernels
-    for (DevBuf *b : {&ctx->hpwl_part, &ctx->ovf_part, &ctx->energy_part, &ctx->part_a, &ctx->part_b})
+    TRY(ctx->keys.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(C, 1)));
+    TRY(ctx->perm.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(C, 1)));
+    TRY(ctx->perm_out.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(C, 1)));
+    TRY(ctx->rec.alloc(sizeof(float4) * (size_t)std::max<int64_t>(C, 1)));
+    TRY(ctx->rratio.alloc(sizeof(float) * (size_t)std::max<int64_t>(C, 1)));
+    TRY(launch_iota(ctx->perm.as<int32_t>(), C, s));
+    TRY(launch_iota(ctx->perm_out.as<int32_t>(), C, s));
+    TRY(ctx->hist.alloc(sizeof(int32_t) * (size_t)kSortTiles));
+    TRY(ctx->cursor.alloc(sizeof(int32_t) * (size_t)kSortTiles));
+    {
+        SortBufs &sb = ctx->sb;
+        sb.num_cells = C;
+        sb.pos = ctx->pos.as<float2>();
+        sb.csize = ctx->csize.as<float2>();
+        sb.coff = ctx->coff.as<float2>();
+        sb.ratio = ctx->ratio.as<float>();
+        sb.m = ctx->m;
+        sb.n = ctx->n;
+        sb.bw = ctx->bw;
+        sb.bh = ctx->bh;
+        sb.keys = ctx->keys.as<int32_t>();
+        sb.hist = ctx->hist.as<int32_t>();
+        sb.cursor = ctx->cursor.as<int32_t>();
+        sb.perm = ctx->perm.as<int32_t>();
+        sb.perm_out = ctx->perm_out.as<int32_t>();
+        sb.rec = ctx->rec.as<float4>();
+        sb.rratio = ctx->rratio.as<float>();
+    }
+    for (DevBuf *b : {&ctx->hpwl_part, &ctx->ovf_part, &ctx->energy_part, &ctx->part_a, &ctx->part_b, &ctx->hist})
         if (hipMemsetAsync(b->p, 0, b->bytes, s) != hipSuccess) { erp_ctx_destroy(ctx); return fail(ERP_ERR_HIP, "hipMemsetAsync failed"); }
     TRY(launch_setup_cells(ctx->d_w.as<float>(), ctx->d_h.as<float>(), ctx->d_npins_all.as<float>(),
                            N, C, ctx->bw, ctx->bh, ctx->t, die_w, die_h, ctx->csize.as<float2>(),
@@ -793,8 +824,8 @@ int erp_final_hpwl_dbu(erp_ctx *ctx, int64_t *hpwl_dbu) {
 
 int erp_set_timing(erp_ctx *ctx, int enabled) {
     if (!ctx) return fail(ERP_ERR_INVALID, "null context");
+    if (enabled && !ctx->timing && ctx->timer) ctx->timer->reset_totals();
     ctx->timing = enabled != 0;
-    if (ctx->timer) ctx->timer->reset_totals();
     return ERP_OK;
 }
 
@@ -879,16 +910,20 @@ int erp_compute_density_map(int64_t nc, const float *x, const float *y, const fl
     pack2(p, x, y, nc);
     pack2(sz, cw, ch, nc);
     pack2(of, ox, oy, nc);
-    DevBuf dp, ds, dof, dr, dq, dinit, dmap;
+    DevBuf dp, ds, dof, dr, dq, dinit, dmap, drec;
     int rc;
     if ((rc = upload(dp, p.data(), p.size(), L.s)) || (rc = upload(ds, sz.data(), sz.size(), L.s)) ||
         (rc = upload(dof, of.data(), of.size(), L.s)) || (rc = upload(dr, ratio, (size_t)nc, L.s)) ||
-        (rc = dq.alloc(sizeof(int64_t) * B)) || (rc = dmap.alloc(sizeof(float) * B)))
+        (rc = dq.alloc(sizeof(int64_t) * B)) || (rc = dmap.alloc(sizeof(float) * B)) ||
+        (rc = drec.alloc(sizeof(float4) * (size_t)std::max<int64_t>(nc, 1))))
         return rc;
     if (initial && (rc = upload(dinit, initial, B, L.s))) return rc;
     if ((rc = launch_zero_i64(dq.as<int64_t>(), (int64_t)B, L.s))) return rc;
-    if ((rc = launch_scatter(dp.as<float2>(), ds.as<float2>(), dof.as<float2>(), dr.as<float>(), nc,
-                             m, n, bw, bh, scale, dq.as<int64_t>(), L.s)))
+    if ((rc = launch_pack_records(dp.as<float2>(), ds.as<float2>(), dof.as<float2>(), nc,
+                                  drec.as<float4>(), L.s)))
+        return rc;
+    if ((rc = launch_scatter(drec.as<float4>(), dr.as<float>(), nc, m, n, bw, bh, scale, nullptr,
+                             nullptr, dq.as<int64_t>(), L.s)))
         return rc;
     if ((rc = launch_finalize_leaf(dq.as<int64_t>(), initial ? dinit.as<float>() : nullptr,
                                    dmap.as<float>(), (int64_t)B, 1.0 / scale, L.s)))

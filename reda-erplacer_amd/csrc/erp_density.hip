@@ -8,8 +8,12 @@
 // (this file is compiled with -ffp-contract=off), rounded once to a multiple of
 // 1/S and added with an int64 atomic: integer addition is associative, so the
 // map is bit-identical for any launch order and equal to a CPU restatement.
+#include <algorithm>
+#include <climits>
+
 #include "erp_internal.h"
 #include "erp_kernels.h"
+#include "erp_timer.h"
 
 namespace erp {
 namespace {
@@ -43,60 +47,255 @@ __device__ __forceinline__ int64_t quantize(float c, double scale) {
 __device__ __forceinline__ void add_q(int64_t *q, int64_t v) {
     atomicAdd(reinterpret_cast<unsigned long long *>(q), (unsigned long long)v);
 }
-__device__ __forceinline__ int64_t wave_sum(int64_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
-}
 
-// Movable scatter: one thread per cell.  When every lane of a wavefront covers
-// the same bin window (all cells start at the die centre, placement.rs:129-130,
-// and stay clustered for many iterations) the wave sums its contributions per
-// bin and issues one atomic per bin instead of 64.
+// ------------------------------------------------------------------ movable scatter
+// Cells are processed in a spatially sorted order, 1024 per workgroup.  rec
+// holds each cell's clamped lower-left corner (x + ox, y + oy) and clamped size,
+// rratio its ratio, both in that order (written by the sort below).  A
+// workgroup accumulates into an LDS int64 tile of kScatterTile bins placed over
+// its cells; each touched tile bin then costs one global atomic per workgroup
+// instead of one per contribution.  A cell whose window does not fit the tile
+// adds its contributions with global atomics directly.  Global 64-bit atomics
+// are the expensive resource here: they are coherent device-wide and far
+// slower than LDS atomics.
 __global__ void __launch_bounds__(kBlock) k_scatter(
-    const float2 *__restrict__ pos, const float2 *__restrict__ csize,
-    const float2 *__restrict__ coff, const float *__restrict__ ratio, int64_t num_cells,
+    const float4 *__restrict__ rec, const float *__restrict__ rratio, int64_t num_cells,
     int m, int n, float bw, float bh, float inv_bw, float inv_bh, double scale,
+    const int32_t *__restrict__ perm_src, int32_t *__restrict__ perm_dst,
     int64_t *__restrict__ q) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < num_cells;
-    float xi = 0.f, yi = 0.f, sx = 0.f, sy = 0.f, r = 0.f;
-    int bxl = 0, bxh = 0, byl = 0, byh = 0;
-    if (valid) {
-        const float2 p = pos[i], s = csize[i], o = coff[i];
-        xi = p.x + o.x;
-        yi = p.y + o.y;
-        sx = s.x;
-        sy = s.y;
-        r = ratio[i];
-        bin_range(xi, sx, inv_bw, m, bxl, bxh);
-        bin_range(yi, sy, inv_bh, n, byl, byh);
-    }
-    const int l_bxl = __shfl(bxl, 0), l_bxh = __shfl(bxh, 0);
-    const int l_byl = __shfl(byl, 0), l_byh = __shfl(byh, 0);
-    const bool same = !valid || (bxl == l_bxl && bxh == l_bxh && byl == l_byl && byh == l_byh);
-    if (__all(same)) {
-        if (!__shfl(valid ? 1 : 0, 0)) return;   // whole wave past the end
-        for (int bx = l_bxl; bx < l_bxh; ++bx) {
-            const float px = r * tri_overlap(xi, sx, bx, bw);
-            for (int by = l_byl; by < l_byh; ++by) {
-                const float py = tri_overlap(yi, sy, by, bh);
-                int64_t v = valid ? quantize(px * py, scale) : 0;
-                v = wave_sum(v);
-                if ((threadIdx.x & 63) == 0 && v != 0) add_q(q + (size_t)bx * n + by, v);
+    __shared__ unsigned long long tile[kScatterTile];
+    __shared__ int wbox[4][kBlock / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScatterCells;
+    float xi[kScatterPer], yi[kScatterPer], sx[kScatterPer], sy[kScatterPer], r[kScatterPer];
+    int bxl[kScatterPer], bxh[kScatterPer], byl[kScatterPer], byh[kScatterPer];
+    int lx = INT_MAX, hx = INT_MIN, ly = INT_MAX, hy = INT_MIN;
+#pragma unroll
+    for (int j = 0; j < kScatterPer; ++j) {
+        const int64_t i = base + threadIdx.x + (int64_t)j * kBlock;
+        bxl[j] = bxh[j] = byl[j] = byh[j] = 0;
+        xi[j] = yi[j] = sx[j] = sy[j] = r[j] = 0.f;
+        if (i < num_cells) {
+            if (perm_dst) perm_dst[i] = perm_src[i];
+            const float4 v = rec[i];
+            xi[j] = v.x;
+            yi[j] = v.y;
+            sx[j] = v.z;
+            sy[j] = v.w;
+            r[j] = rratio[i];
+            bin_range(xi[j], sx[j], inv_bw, m, bxl[j], bxh[j]);
+            bin_range(yi[j], sy[j], inv_bh, n, byl[j], byh[j]);
+            if (bxh[j] > bxl[j] && byh[j] > byl[j]) {
+                lx = min(lx, bxl[j]); hx = max(hx, bxh[j]);
+                ly = min(ly, byl[j]); hy = max(hy, byh[j]);
             }
         }
-        return;
     }
-    if (!valid) return;
-    for (int bx = bxl; bx < bxh; ++bx) {
-        const float px = r * tri_overlap(xi, sx, bx, bw);
-        for (int by = byl; by < byh; ++by) {
-            const float py = tri_overlap(yi, sy, by, bh);
-            const int64_t v = quantize(px * py, scale);
-            if (v != 0) add_q(q + (size_t)bx * n + by, v);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lx = min(lx, __shfl_xor(lx, off)); hx = max(hx, __shfl_xor(hx, off));
+        ly = min(ly, __shfl_xor(ly, off)); hy = max(hy, __shfl_xor(hy, off));
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        wbox[0][wave] = lx; wbox[1][wave] = hx; wbox[2][wave] = ly; wbox[3][wave] = hy;
+    }
+    __syncthreads();
+    lx = wbox[0][0]; hx = wbox[1][0]; ly = wbox[2][0]; hy = wbox[3][0];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) {
+        lx = min(lx, wbox[0][w]); hx = max(hx, wbox[1][w]);
+        ly = min(ly, wbox[2][w]); hy = max(hy, wbox[3][w]);
+    }
+    if (hx <= lx || hy <= ly) return;      // no contribution in this workgroup (uniform)
+    // LDS tile: the bounding box when it fits, else a kScatterSide^2 window
+    // centred on the box (cells outside it go to global memory directly)
+    int tx0 = lx, ty0 = ly, wx = hx - lx, wy = hy - ly;
+    if ((int64_t)wx * wy > kScatterTile) {
+        wx = min(wx, kScatterSide);
+        wy = min(wy, kScatterTile / wx);
+        tx0 = (lx + hx - wx) / 2;
+        ty0 = (ly + hy - wy) / 2;
+    }
+    const int area = wx * wy;
+    for (int t = threadIdx.x; t < area; t += kBlock) tile[t] = 0ull;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kScatterPer; ++j) {
+        const bool inside = bxl[j] >= tx0 && bxh[j] <= tx0 + wx && byl[j] >= ty0 && byh[j] <= ty0 + wy;
+        for (int bx = bxl[j]; bx < bxh[j]; ++bx) {
+            const float px = r[j] * tri_overlap(xi[j], sx[j], bx, bw);
+            for (int by = byl[j]; by < byh[j]; ++by) {
+                const float py = tri_overlap(yi[j], sy[j], by, bh);
+                const int64_t v = quantize(px * py, scale);
+                if (v == 0) continue;
+                if (inside)
+                    atomicAdd(&tile[(bx - tx0) * wy + (by - ty0)], (unsigned long long)v);
+                else
+                    add_q(q + (size_t)bx * n + by, v);
+            }
         }
     }
+    __syncthreads();
+    for (int t = threadIdx.x; t < area; t += kBlock) {
+        const unsigned long long v = tile[t];
+        if (v != 0ull) add_q(q + (size_t)(tx0 + t / wy) * n + (ty0 + t % wy), (int64_t)v);
+    }
+}
+
+// ------------------------------------------------------- spatial counting sort
+// Every iteration the cells are re-bucketed by the sort tile (tiles of 16x16
+// bins, coarser on grids above 1024, at most kSortTiles tiles, in Morton order
+// so that consecutive tiles are close) of their clamped lower-left corner.  The
+// input order is the previous iteration's sorted order, so each workgroup sees
+// cells of only a few tiles and the global histogram / reservation atomics stay
+// few.  Pass 1: keys + histogram.  Pass 2: exclusive scan into cursors (and a
+// cleared histogram for the next sort).  Pass 3: per-workgroup ranks in LDS, one
+// reservation atomic per tile and workgroup, and the sorted scatter records.
+// Order inside a tile depends on the schedule; the fixed-point scatter is order
+// independent, so the density map is not.
+__device__ __forceinline__ unsigned spread_bits(unsigned v) {   // 0b abc -> 0b a0b0c
+    v &= 0xffu;
+    v = (v | (v << 4)) & 0x0f0fu;
+    v = (v | (v << 2)) & 0x3333u;
+    v = (v | (v << 1)) & 0x5555u;
+    return v;
+}
+__device__ __forceinline__ int sort_key(float lx, float ly, float inv_bw, float inv_bh, int m,
+                                        int n, int shift) {
+    int bx = (int)floorf(lx * inv_bw), by = (int)floorf(ly * inv_bh);
+    bx = bx < 0 ? 0 : (bx > m - 1 ? m - 1 : bx);
+    by = by < 0 ? 0 : (by > n - 1 ? n - 1 : by);
+    return (int)((spread_bits((unsigned)(bx >> shift)) << 1) | spread_bits((unsigned)(by >> shift)));
+}
+
+__global__ void __launch_bounds__(kBlock) k_sort_keys(const int32_t *__restrict__ perm,
+                                                      const float2 *__restrict__ pos,
+                                                      const float2 *__restrict__ coff,
+                                                      int64_t num_cells, float inv_bw,
+                                                      float inv_bh, int m, int n, int shift,
+                                                      int ntiles, int32_t *__restrict__ keys,
+                                                      int32_t *__restrict__ hist) {
+    __shared__ int lh[kSortTiles];
+    for (int t = threadIdx.x; t < ntiles; t += kBlock) lh[t] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kScatterCells;
+#pragma unroll
+    for (int j = 0; j < kScatterPer; ++j) {
+        const int64_t i = base + threadIdx.x + (int64_t)j * kBlock;
+        const bool valid = i < num_cells;
+        int k = -1;
+        if (valid) {
+            const int32_t c = perm[i];
+            const float2 p = pos[c], o = coff[c];
+            k = sort_key(p.x + o.x, p.y + o.y, inv_bw, inv_bh, m, n, shift);
+            keys[i] = k;
+        }
+        // a wavefront whose cells share one tile adds its count once
+        const int k0 = __shfl(k, 0);
+        if (__all(k == k0 || !valid)) {
+            const int cnt = __popcll(__ballot(valid));
+            if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(&lh[k0], cnt);
+        } else if (valid) {
+            atomicAdd(&lh[k], 1);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < ntiles; t += kBlock)
+        if (lh[t]) atomicAdd(&hist[t], lh[t]);
+}
+
+__global__ void __launch_bounds__(1024) k_sort_scan(int32_t *__restrict__ hist,
+                                                    int32_t *__restrict__ cursor, int ntiles) {
+    __shared__ int part[1024];
+    const int per = (ntiles + 1023) / 1024;
+    const int t0 = threadIdx.x * per;
+    int sum = 0;
+    for (int k = 0; k < per; ++k)
+        if (t0 + k < ntiles) sum += hist[t0 + k];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {      // inclusive Hillis-Steele scan
+        const int v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = part[threadIdx.x] - sum;              // exclusive prefix of this segment
+    for (int k = 0; k < per; ++k) {
+        if (t0 + k < ntiles) {
+            cursor[t0 + k] = run;
+            run += hist[t0 + k];
+            hist[t0 + k] = 0;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_sort_place(const int32_t *__restrict__ perm,
+                                                       const int32_t *__restrict__ keys,
+                                                       const float2 *__restrict__ pos,
+                                                       const float2 *__restrict__ csize,
+                                                       const float2 *__restrict__ coff,
+                                                       const float *__restrict__ ratio,
+                                                       int64_t num_cells, int ntiles,
+                                                       int32_t *__restrict__ cursor,
+                                                       int32_t *__restrict__ perm_out,
+                                                       float4 *__restrict__ rec,
+                                                       float *__restrict__ rratio) {
+    __shared__ int lh[kSortTiles];
+    for (int t = threadIdx.x; t < ntiles; t += kBlock) lh[t] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kScatterCells;
+    int key[kScatterPer], rank[kScatterPer];
+#pragma unroll
+    for (int j = 0; j < kScatterPer; ++j) {
+        const int64_t i = base + threadIdx.x + (int64_t)j * kBlock;
+        const bool valid = i < num_cells;
+        key[j] = valid ? keys[i] : -1;
+        rank[j] = 0;
+        // valid lanes are a prefix of the wavefront; if they share one tile the
+        // wavefront reserves its ranks with one LDS atomic
+        const int k0 = __shfl(key[j], 0);
+        if (__all(key[j] == k0 || !valid)) {
+            const int cnt = __popcll(__ballot(valid));
+            int b0 = 0;
+            if ((threadIdx.x & 63) == 0 && cnt) b0 = atomicAdd(&lh[k0], cnt);
+            rank[j] = __shfl(b0, 0) + (int)(threadIdx.x & 63);
+        } else if (valid) {
+            rank[j] = atomicAdd(&lh[key[j]], 1);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < ntiles; t += kBlock)
+        if (lh[t]) lh[t] = atomicAdd(&cursor[t], lh[t]);   // count -> reserved start
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kScatterPer; ++j) {
+        if (key[j] >= 0) {
+            const int64_t i = base + threadIdx.x + (int64_t)j * kBlock;
+            const int dst = lh[key[j]] + rank[j];
+            const int32_t c = perm[i];
+            const float2 p = pos[c], sz = csize[c], o = coff[c];
+            perm_out[dst] = c;
+            rec[dst] = make_float4(p.x + o.x, p.y + o.y, sz.x, sz.y);
+            rratio[dst] = ratio[c];
+        }
+    }
+}
+
+// Scatter records in identity order (leaf entry point, no sort).
+__global__ void k_pack_records(const float2 *__restrict__ pos, const float2 *__restrict__ csize,
+                               const float2 *__restrict__ coff, int64_t num_cells,
+                               float4 *__restrict__ rec) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < num_cells) {
+        const float2 p = pos[i], sz = csize[i], o = coff[i];
+        rec[i] = make_float4(p.x + o.x, p.y + o.y, sz.x, sz.y);
+    }
+}
+
+__global__ void k_iota(int32_t *__restrict__ v, int64_t count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) v[i] = (int32_t)i;
 }
 
 // Fixed cells (density.rs:26-99): one workgroup per cell over its bin window,
@@ -147,13 +346,68 @@ inline unsigned grid_for(int64_t count) { return (unsigned)((count + kBlock - 1)
 
 }  // namespace
 
-int launch_scatter(const float2 *pos, const float2 *csize, const float2 *coff, const float *ratio,
-                   int64_t num_cells, int m, int n, float bw, float bh, double scale, int64_t *q,
-                   hipStream_t s) {
+int launch_scatter(const float4 *rec, const float *rratio, int64_t num_cells, int m, int n,
+                   float bw, float bh, double scale, const int32_t *perm_src, int32_t *perm_dst,
+                   int64_t *q, hipStream_t s) {
     if (num_cells <= 0) return ERP_OK;
     const float inv_bw = 1.f / bw, inv_bh = 1.f / bh;
-    hipLaunchKernelGGL(k_scatter, dim3(grid_for(num_cells)), dim3(kBlock), 0, s, pos, csize, coff,
-                       ratio, num_cells, m, n, bw, bh, inv_bw, inv_bh, scale, q);
+    const unsigned nb = (unsigned)((num_cells + kScatterCells - 1) / kScatterCells);
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, s, rec, rratio, num_cells, m, n, bw,
+                       bh, inv_bw, inv_bh, scale, perm_src, perm_dst, q);
+    ERP_HIP_TRY(hipGetLastError());
+    return ERP_OK;
+}
+
+SortGeom sort_geometry(int m, int n) {
+    SortGeom g;
+    // Morton keys over a square of side^2 tiles, side = next power of two of the
+    // larger tile count; coarsen the tiles until that square fits kSortTiles.
+    for (g.shift = 4;; ++g.shift) {
+        const int tx = std::max(m >> g.shift, 1), ty = std::max(n >> g.shift, 1);
+        int side = 1;
+        while (side < tx || side < ty) side <<= 1;
+        g.ntiles = side * side;
+        if (g.ntiles <= kSortTiles) break;
+    }
+    return g;
+}
+
+int launch_sort(const SortBufs &b, hipStream_t s, KernelTimer *t) {
+    if (b.num_cells <= 0) return ERP_OK;
+    const SortGeom g = sort_geometry(b.m, b.n);
+    const unsigned nb = (unsigned)((b.num_cells + kScatterCells - 1) / kScatterCells);
+    const float inv_bw = 1.f / b.bw, inv_bh = 1.f / b.bh;
+    if (t) t->begin("sort_keys");
+    hipLaunchKernelGGL(k_sort_keys, dim3(nb), dim3(kBlock), 0, s, b.perm, b.pos, b.coff,
+                       b.num_cells, inv_bw, inv_bh, b.m, b.n, g.shift, g.ntiles, b.keys, b.hist);
+    if (t) t->end();
+    ERP_HIP_TRY(hipGetLastError());
+    if (t) t->begin("sort_scan");
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, b.hist, b.cursor, g.ntiles);
+    if (t) t->end();
+    ERP_HIP_TRY(hipGetLastError());
+    if (t) t->begin("sort_place");
+    hipLaunchKernelGGL(k_sort_place, dim3(nb), dim3(kBlock), 0, s, b.perm, b.keys, b.pos, b.csize,
+                       b.coff, b.ratio, b.num_cells, g.ntiles, b.cursor, b.perm_out, b.rec,
+                       b.rratio);
+    if (t) t->end();
+    ERP_HIP_TRY(hipGetLastError());
+    return ERP_OK;
+}
+
+int launch_pack_records(const float2 *pos, const float2 *csize, const float2 *coff,
+                        int64_t num_cells, float4 *rec, hipStream_t s) {
+    if (num_cells <= 0) return ERP_OK;
+    hipLaunchKernelGGL(k_pack_records, dim3((unsigned)((num_cells + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, pos, csize, coff, num_cells, rec);
+    ERP_HIP_TRY(hipGetLastError());
+    return ERP_OK;
+}
+
+int launch_iota(int32_t *v, int64_t count, hipStream_t s) {
+    if (count <= 0) return ERP_OK;
+    hipLaunchKernelGGL(k_iota, dim3((unsigned)((count + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       v, count);
     ERP_HIP_TRY(hipGetLastError());
     return ERP_OK;
 }

