@@ -8,8 +8,7 @@
 #include <mutex>
 #include <numeric>
 
-#include "erp_internal.h"
-#include "erp_kernels.h"
+#include "erp_ctx.h"
 #include "erp_timer.h"
 
 namespace erp {
@@ -33,42 +32,6 @@ double fx_scale(double mass_bound) {
 }  // namespace erp
 
 using namespace erp;
-
-struct erp_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int64_t N = 0, C = 0, F = 0, P = 0, E = 0, n2 = 0, Pm = 0;
-    int m = 0, n = 0;
-    int64_t B = 0;
-    float bw = 0, bh = 0, die_w = 0, die_h = 0, t = 0;
-    double movable_area = 0, fx_scale = 1, coef = 0, gamma0 = 0, dw0 = 0;
-    int64_t coef_q = 0;
-    int spectral_mode = ERP_SPECTRAL_FFT;
-    int64_t iter_host = 0;
-    bool debug_parts = true;   // keep per-cell WA / density gradients in the fused loop
-    bool keep_q = false;       // keep the movable accumulators after pass 1
-    bool q_dirty = false;
-    // device buffers
-    DevBuf d_x, d_y, d_w, d_h, d_npins_all;
-    DevBuf pos, prev_pos, prev_g, g, gwl, gd;
-    DevBuf csize, coff, ratio, area, npins, hi;
-    DevBuf slot_node, slot_off, net_start, slot_grad, cstart, cslots;
-    DevBuf q_mov, q_fix, rho, rowt, p1, p2, p3, fx, fy, pot, auv;
-    DevBuf hpwl_part, ovf_part, energy_part, part_a, part_b;
-    DevBuf state, gamma_override, hpwl_dbu;
-    DevBuf keys, perm, perm_out, hist, cursor, rec, rratio;
-    SortBufs sb{};
-    SpecTables tm, tn;
-    MfmaPlan *mfma = nullptr;
-    WaNets wa{};
-    CellBufs cb{};
-    PotentialBufs pb{};
-    ScalarArgs sa{};
-    int n_hpwl_part = 0, n_pot_part = 0, n_cell_part = 0;
-    hipGraphExec_t graph_many = nullptr, graph_one = nullptr;
-    KernelTimer *timer = nullptr;
-    bool timing = false;
-};
 
 namespace {
 
@@ -119,12 +82,27 @@ int timer_kernel_names(erp_ctx *ctx, double *ms, int32_t cap, int32_t *count, co
 }
 
 // ---------------------------------------------------------------- phases
-int phase_hpwl(erp_ctx *ctx, const double *gamma_dev) {
+const JobDesc *djob(erp_ctx *ctx) { return ctx->djob.as<JobDesc>(); }
+
+}  // namespace
+
+int erp::upload_job(erp_ctx *ctx) {
+    ERP_HIP_TRY(hipMemcpyAsync(ctx->djob.p, &ctx->job, sizeof(JobDesc), hipMemcpyHostToDevice, ctx->stream));
+    JobDesc g = ctx->job;
+    g.wa.gamma = ctx->gamma_override.as<double>();
+    ERP_HIP_TRY(hipMemcpyAsync(ctx->djob_gamma.p, &g, sizeof(JobDesc), hipMemcpyHostToDevice, ctx->stream));
+    ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return ERP_OK;
+}
+
+namespace {
+
+int phase_hpwl(erp_ctx *ctx, const JobDesc *d) {
     KernelTimer *t = ctx->timing ? ctx->timer : nullptr;
-    int rc = launch_wa(ctx->wa, ctx->pos.as<float2>(), gamma_dev, ctx->stream, t);
+    int rc = launch_wa(d, ctx->dims, 1, ctx->stream, t);
     if (rc) return rc;
-    if ((rc = launch_pin2cell(ctx->cb, ctx->stream))) return rc;
-    return launch_scalar(ctx->sa, SC_HPWL, ctx->stream);
+    if ((rc = launch_pin2cell(d, ctx->dims, 1, ctx->stream))) return rc;
+    return launch_scalar(d, 1, SC_HPWL, ctx->stream);
 }
 
 int run_potential(erp_ctx *ctx) {
@@ -134,19 +112,15 @@ int run_potential(erp_ctx *ctx) {
         if ((rc = launch_zero_i64(ctx->q_mov.as<int64_t>(), ctx->B, ctx->stream))) return rc;
         ctx->q_dirty = false;
     }
-    if ((rc = launch_sort(ctx->sb, ctx->stream, t))) return rc;
+    if ((rc = launch_sort(djob(ctx), ctx->dims, 1, ctx->stream, t))) return rc;
     if (t) t->begin("scatter");
-    rc = launch_scatter(ctx->rec.as<float4>(), ctx->rratio.as<float>(), ctx->C, ctx->m, ctx->n,
-                        ctx->bw, ctx->bh, ctx->fx_scale, ctx->perm_out.as<int32_t>(),
-                        ctx->perm.as<int32_t>(), ctx->q_mov.as<int64_t>(), ctx->stream);
+    rc = launch_scatter(djob(ctx), ctx->dims, 1, true, ctx->stream);
     if (t) t->end();
     if (rc) return rc;
-    PotentialBufs b = ctx->pb;
-    b.keep_q = ctx->keep_q;
     if (ctx->spectral_mode == ERP_SPECTRAL_MFMA_BF16X3)
-        rc = launch_potential_mfma(b, ctx->mfma, ctx->stream, t);
+        rc = launch_potential_mfma(ctx->job.pb, ctx->mfma, ctx->stream, t);
     else
-        rc = launch_potential_fft(b, ctx->tm, ctx->tn, ctx->stream, t);
+        rc = launch_potential_fft(djob(ctx), ctx->dims, 1, ctx->stream, t);
     if (ctx->keep_q) ctx->q_dirty = true;
     return rc;
 }
@@ -154,24 +128,23 @@ int run_potential(erp_ctx *ctx) {
 int phase_potential(erp_ctx *ctx) {
     int rc = run_potential(ctx);
     if (rc) return rc;
-    if ((rc = launch_efield(ctx->cb, ctx->stream))) return rc;
-    return launch_scalar(ctx->sa, SC_POT, ctx->stream);
+    if ((rc = launch_efield(djob(ctx), ctx->dims, 1, ctx->stream))) return rc;
+    return launch_scalar(djob(ctx), 1, SC_POT, ctx->stream);
 }
 
 int phase_nesterov(erp_ctx *ctx) {
-    DevState *st = ctx->state.as<DevState>();
     int rc;
     if (ctx->iter_host == 0) {
-        if ((rc = launch_l1(ctx->cb, ctx->stream))) return rc;
-        if ((rc = launch_scalar(ctx->sa, SC_L1, ctx->stream))) return rc;
+        if ((rc = launch_l1(djob(ctx), ctx->dims, 1, ctx->stream))) return rc;
+        if ((rc = launch_scalar(djob(ctx), 1, SC_L1, ctx->stream))) return rc;
     }
-    if ((rc = launch_combine(ctx->cb, st, ctx->stream))) return rc;
-    if ((rc = launch_scalar(ctx->sa, SC_NEST, ctx->stream))) return rc;
-    return launch_update(ctx->cb, st, ctx->stream);
+    if ((rc = launch_combine(djob(ctx), ctx->dims, 1, ctx->stream))) return rc;
+    if ((rc = launch_scalar(djob(ctx), 1, SC_NEST, ctx->stream))) return rc;
+    return launch_update(djob(ctx), ctx->dims, 1, ctx->stream);
 }
 
 int phase_scheduler(erp_ctx *ctx) {
-    int rc = launch_scalar(ctx->sa, SC_SCHED, ctx->stream);
+    int rc = launch_scalar(djob(ctx), 1, SC_SCHED, ctx->stream);
     if (rc == ERP_OK) ctx->iter_host += 1;
     return rc;
 }
@@ -179,20 +152,19 @@ int phase_scheduler(erp_ctx *ctx) {
 // One iteration (iteration >= 1) as a fused kernel sequence: what the graphs hold.
 int launch_fused_iteration(erp_ctx *ctx) {
     KernelTimer *t = ctx->timing ? ctx->timer : nullptr;
-    DevState *st = ctx->state.as<DevState>();
-    int rc = launch_wa(ctx->wa, ctx->pos.as<float2>(), &st->gamma, ctx->stream, t);
+    int rc = launch_wa(djob(ctx), ctx->dims, 1, ctx->stream, t);
     if (rc) return rc;
     if ((rc = run_potential(ctx))) return rc;
     if (t) t->begin("cell_fused");
-    rc = launch_cell_fused(ctx->cb, st, ctx->debug_parts, ctx->stream);
+    rc = launch_cell_fused(djob(ctx), ctx->dims, 1, ctx->debug_parts, ctx->stream);
     if (t) t->end();
     if (rc) return rc;
     if (t) t->begin("scalar");
-    rc = launch_scalar(ctx->sa, SC_HPWL | SC_POT | SC_NEST | SC_SCHED, ctx->stream);
+    rc = launch_scalar(djob(ctx), 1, SC_HPWL | SC_POT | SC_NEST | SC_SCHED, ctx->stream);
     if (t) t->end();
     if (rc) return rc;
     if (t) t->begin("update");
-    rc = launch_update(ctx->cb, st, ctx->stream);
+    rc = launch_update(djob(ctx), ctx->dims, 1, ctx->stream);
     if (t) t->end();
     return rc;
 }
@@ -217,7 +189,9 @@ int capture(erp_ctx *ctx, int iters, hipGraphExec_t *out) {
     return ERP_OK;
 }
 
-int read_info(erp_ctx *ctx, erp_step_info *info) {
+}  // namespace
+
+int erp::read_info(erp_ctx *ctx, erp_step_info *info) {
     if (!info) return ERP_OK;
     DevState s;
     ERP_HIP_TRY(hipMemcpyAsync(&s, ctx->state.p, sizeof(DevState), hipMemcpyDeviceToHost, ctx->stream));
@@ -235,6 +209,8 @@ int read_info(erp_ctx *ctx, erp_step_info *info) {
     return ERP_OK;
 }
 
+namespace {
+
 int reset_ctx(erp_ctx *ctx) {
     int rc = launch_init_positions(ctx->pos.as<float2>(), ctx->d_w.as<float>(), ctx->d_h.as<float>(),
                                    ctx->d_x.as<float>(), ctx->d_y.as<float>(), ctx->N, ctx->C,
@@ -250,6 +226,45 @@ int reset_ctx(erp_ctx *ctx) {
 }
 
 }  // namespace
+
+int erp::launch_batched_iteration(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s) {
+    int rc = launch_wa(d, bd, nj, s);
+    if (!rc) rc = launch_sort(d, bd, nj, s);
+    if (!rc) rc = launch_scatter(d, bd, nj, true, s);
+    if (!rc) rc = launch_potential_fft(d, bd, nj, s);
+    if (!rc) rc = launch_cell_fused(d, bd, nj, false, s);
+    if (!rc) rc = launch_scalar(d, nj, SC_HPWL | SC_POT | SC_NEST | SC_SCHED, s);
+    if (!rc) rc = launch_update(d, bd, nj, s);
+    return rc;
+}
+
+size_t erp::estimate_ctx_bytes(const erp_design *d, const erp_config *cfg) {
+    const size_t N = (size_t)d->num_nodes, C = (size_t)d->num_movable, P = (size_t)d->num_pins_total;
+    const size_t E = (size_t)d->num_nets, B = (size_t)cfg->num_bins_x * (size_t)cfg->num_bins_y;
+    const size_t per_node = 5 * 4 + 3 * 8 + 4;                       // DB copies, pos/csize/coff, ratio
+    const size_t per_cell = 6 * 8 + 2 * 4 + 8 + 4 * 4 + 16 + 4 + 4;  // state, area/pins, hi, sort
+    const size_t per_pin = 4 + 8 + 8 + 4;                             // slots, grads, cell->slot
+    const size_t per_bin = 2 * 8 + 9 * 4;                             // q_mov/q_fix + float maps
+    const size_t partials = 16 * (P / 64 + C / 128 + (size_t)cfg->num_bins_x) + 64 * 1024;
+    return N * per_node + C * per_cell + P * per_pin + E * 4 + B * per_bin + partials +
+           2 * 8 * (size_t)(cfg->num_bins_x + cfg->num_bins_y) + (1u << 20);
+}
+
+size_t erp::ctx_device_bytes(const erp_ctx *c) {
+    const DevBuf *bufs[] = {&c->d_x, &c->d_y, &c->d_w, &c->d_h, &c->d_npins_all, &c->pos,
+                            &c->prev_pos, &c->prev_g, &c->g, &c->gwl, &c->gd, &c->csize, &c->coff,
+                            &c->ratio, &c->area, &c->npins, &c->hi, &c->slot_node, &c->slot_off,
+                            &c->net_start, &c->slot_grad, &c->cstart, &c->cslots, &c->q_mov,
+                            &c->q_fix, &c->rho, &c->rowt, &c->p1, &c->p2, &c->p3, &c->fx, &c->fy,
+                            &c->pot, &c->auv, &c->hpwl_part, &c->ovf_part, &c->energy_part,
+                            &c->part_a, &c->part_b, &c->state, &c->gamma_override, &c->hpwl_dbu,
+                            &c->keys, &c->perm, &c->perm_out, &c->hist, &c->cursor, &c->rec,
+                            &c->rratio, &c->djob, &c->djob_gamma, &c->tm.fft_tw, &c->tm.dct_tw,
+                            &c->tn.fft_tw, &c->tn.dct_tw};
+    size_t total = 0;
+    for (const DevBuf *b : bufs) total += b->bytes;
+    return total;
+}
 
 extern "C" {
 
@@ -413,7 +428,7 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     TRY(ctx->hpwl_dbu.alloc(sizeof(unsigned long long)));
 
     // ------------------------------------------------------------- kernel argument blocks
-    WaNets &w = ctx->wa;
+    WaNets &w = ctx->job.wa;
     w.slot_node = ctx->slot_node.as<int32_t>();
     w.slot_off = ctx->slot_off.as<float2>();
     w.net_start = ctx->net_start.as<int32_t>();
@@ -421,6 +436,8 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     w.bucket_begin[0] = d->n_2nets;
     for (int b = 0; b < kWaBuckets; ++b) w.bucket_begin[b + 1] = w.bucket_begin[b] + bucket_count[b];
     w.slot_grad = ctx->slot_grad.as<float2>();
+    w.pos = ctx->pos.as<float2>();
+    w.gamma = &ctx->state.as<DevState>()->gamma;
     ctx->n_hpwl_part = wa_partials(w);
     TRY(ctx->hpwl_part.alloc(sizeof(double) * (size_t)std::max(ctx->n_hpwl_part, 1)));
     w.hpwl_part = ctx->hpwl_part.as<double>();
@@ -428,7 +445,7 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     ctx->n_pot_part = potential_partials(ctx->m);
     TRY(ctx->ovf_part.alloc(sizeof(int64_t) * (size_t)ctx->n_pot_part));
     TRY(ctx->energy_part.alloc(sizeof(double) * (size_t)ctx->n_pot_part));
-    PotentialBufs &pb = ctx->pb;
+    PotentialBufs &pb = ctx->job.pb;
     pb.q_fix = ctx->q_fix.as<int64_t>();
     pb.q_mov = ctx->q_mov.as<int64_t>();
     pb.rho = ctx->rho.as<float>();
@@ -453,11 +470,12 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     pb.bw = ctx->bw;
     pb.bh = ctx->bh;
     pb.keep_q = false;
+    pb.scale = ctx->fx_scale;
 
     ctx->n_cell_part = cell_partials(C);
     TRY(ctx->part_a.alloc(sizeof(double) * (size_t)ctx->n_cell_part));
     TRY(ctx->part_b.alloc(sizeof(double) * (size_t)ctx->n_cell_part));
-    CellBufs &cb = ctx->cb;
+    CellBufs &cb = ctx->job.cb;
     cb.C = C;
     cb.pos = ctx->pos.as<float2>();
     cb.csize = ctx->csize.as<float2>();
@@ -483,8 +501,9 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     cb.n = ctx->n;
     cb.bw = ctx->bw;
     cb.bh = ctx->bh;
+    cb.st = ctx->state.as<DevState>();
 
-    ScalarArgs &sa = ctx->sa;
+    ScalarArgs &sa = ctx->job.sa;
     sa.st = ctx->state.as<DevState>();
     sa.hpwl_part = ctx->hpwl_part.as<double>();
     sa.n_hpwl = ctx->n_hpwl_part;
@@ -508,7 +527,7 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     TRY(ctx->hist.alloc(sizeof(int32_t) * (size_t)kSortTiles));
     TRY(ctx->cursor.alloc(sizeof(int32_t) * (size_t)kSortTiles));
     {
-        SortBufs &sb = ctx->sb;
+        SortBufs &sb = ctx->job.sb;
         sb.num_cells = C;
         sb.pos = ctx->pos.as<float2>();
         sb.csize = ctx->csize.as<float2>();
@@ -540,6 +559,15 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     TRY(launch_scale_fixed(ctx->q_fix.as<int64_t>(), ctx->B, (double)ctx->t, s));
     TRY(make_spec_tables(ctx->tm, ctx->m, s));
     TRY(make_spec_tables(ctx->tn, ctx->n, s));
+    pb.tm_fft = ctx->tm.fft_tw.as<float2>();
+    pb.tm_dct = ctx->tm.dct_tw.as<float2>();
+    pb.tn_fft = ctx->tn.fft_tw.as<float2>();
+    pb.tn_dct = ctx->tn.dct_tw.as<float2>();
+    TRY(ctx->djob.alloc(sizeof(JobDesc)));
+    TRY(ctx->djob_gamma.alloc(sizeof(JobDesc)));
+    ctx->dims = BatchDims();
+    ctx->dims.add(ctx->job);
+    TRY(upload_job(ctx));
     if (ctx->spectral_mode == ERP_SPECTRAL_MFMA_BF16X3)
         TRY(mfma_plan_create(&ctx->mfma, ctx->m, ctx->n, s));
     ctx->timer = new KernelTimer(s);
@@ -601,7 +629,7 @@ int erp_compute_hpwl(erp_ctx *ctx, double *hpwl) {
     if (!ctx) return fail(ERP_ERR_INVALID, "null context");
     DeviceGuard g(ctx->device);
     if (g.rc) return g.rc;
-    int rc = phase_hpwl(ctx, &ctx->state.as<DevState>()->gamma);
+    int rc = phase_hpwl(ctx, djob(ctx));
     if (rc || !hpwl) return rc;
     return read_scalar(ctx, offsetof(DevState, hpwl), hpwl);
 }
@@ -612,7 +640,7 @@ int erp_compute_hpwl_gamma(erp_ctx *ctx, double gamma, double *hpwl) {
     DeviceGuard g(ctx->device);
     if (g.rc) return g.rc;
     ERP_HIP_TRY(hipMemcpyAsync(ctx->gamma_override.p, &gamma, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    int rc = phase_hpwl(ctx, ctx->gamma_override.as<double>());
+    int rc = phase_hpwl(ctx, ctx->djob_gamma.as<JobDesc>());
     if (rc || !hpwl) return rc;
     return read_scalar(ctx, offsetof(DevState, hpwl), hpwl);
 }
@@ -647,7 +675,7 @@ int erp_step(erp_ctx *ctx, erp_step_info *info) {
     DeviceGuard g(ctx->device);
     if (g.rc) return g.rc;
     if (ctx->timing) ctx->timer->clear();
-    int rc = phase_hpwl(ctx, &ctx->state.as<DevState>()->gamma);
+    int rc = phase_hpwl(ctx, djob(ctx));
     if (!rc) rc = phase_potential(ctx);
     if (!rc) rc = phase_nesterov(ctx);
     if (!rc) rc = phase_scheduler(ctx);
@@ -664,7 +692,7 @@ int erp_run(erp_ctx *ctx, int64_t iterations, erp_step_info *info) {
     int rc = ERP_OK;
     if (ctx->timing) ctx->timer->clear();
     while (iterations > 0 && ctx->iter_host == 0) {   // iteration 0: density-weight scaling
-        rc = phase_hpwl(ctx, &ctx->state.as<DevState>()->gamma);
+        rc = phase_hpwl(ctx, djob(ctx));
         if (!rc) rc = phase_potential(ctx);
         if (!rc) rc = phase_nesterov(ctx);
         if (!rc) rc = phase_scheduler(ctx);
@@ -770,8 +798,11 @@ int erp_get_density_fixed(erp_ctx *ctx, int64_t *q_mov, int64_t *q_fix, double *
 
 int erp_set_keep_density(erp_ctx *ctx, int keep) {
     if (!ctx) return fail(ERP_ERR_INVALID, "null context");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
     ctx->keep_q = keep != 0;
-    return ERP_OK;
+    ctx->job.pb.keep_q = ctx->keep_q;   // read by pass 1 from the device descriptor
+    return upload_job(ctx);
 }
 
 int erp_set_store_gradients(erp_ctx *ctx, int store) {
@@ -814,7 +845,7 @@ int erp_final_hpwl_dbu(erp_ctx *ctx, int64_t *hpwl_dbu) {
     if (g.rc) return g.rc;
     unsigned long long zero = 0, v = 0;
     ERP_HIP_TRY(hipMemcpyAsync(ctx->hpwl_dbu.p, &zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
-    int rc = launch_hpwl_dbu(ctx->wa, ctx->pos.as<float2>(), ctx->E, ctx->hpwl_dbu.as<unsigned long long>(), ctx->stream);
+    int rc = launch_hpwl_dbu(ctx->job.wa, ctx->E, ctx->hpwl_dbu.as<unsigned long long>(), ctx->stream);
     if (rc) return rc;
     ERP_HIP_TRY(hipMemcpyAsync(&v, ctx->hpwl_dbu.p, sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
     ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -922,9 +953,24 @@ int erp_compute_density_map(int64_t nc, const float *x, const float *y, const fl
     if ((rc = launch_pack_records(dp.as<float2>(), ds.as<float2>(), dof.as<float2>(), nc,
                                   drec.as<float4>(), L.s)))
         return rc;
-    if ((rc = launch_scatter(drec.as<float4>(), dr.as<float>(), nc, m, n, bw, bh, scale, nullptr,
-                             nullptr, dq.as<int64_t>(), L.s)))
-        return rc;
+    JobDesc j{};
+    j.sb.num_cells = nc;
+    j.sb.rec = drec.as<float4>();
+    j.sb.rratio = dr.as<float>();
+    j.sb.m = m;
+    j.sb.n = n;
+    j.sb.bw = bw;
+    j.sb.bh = bh;
+    j.pb.q_mov = dq.as<int64_t>();
+    j.pb.scale = scale;
+    j.pb.m = m;
+    j.pb.n = n;
+    j.cb.C = nc;
+    DevBuf dj;
+    if ((rc = upload(dj, &j, 1, L.s))) return rc;
+    BatchDims bd;
+    bd.add(j);
+    if ((rc = launch_scatter(dj.as<JobDesc>(), bd, 1, false, L.s))) return rc;
     if ((rc = launch_finalize_leaf(dq.as<int64_t>(), initial ? dinit.as<float>() : nullptr,
                                    dmap.as<float>(), (int64_t)B, 1.0 / scale, L.s)))
         return rc;
@@ -1005,7 +1051,15 @@ int erp_compute_electric_force(int64_t nc, int32_t m, int32_t n, const float *fi
     c.n = n;
     c.bw = bw;
     c.bh = bh;
-    if ((rc = launch_efield(c, L.s))) return rc;
+    JobDesc j{};
+    j.cb = c;
+    j.pb.m = m;
+    j.pb.n = n;
+    DevBuf dj;
+    if ((rc = upload(dj, &j, 1, L.s))) return rc;
+    BatchDims bd;
+    bd.add(j);
+    if ((rc = launch_efield(dj.as<JobDesc>(), bd, 1, L.s))) return rc;
     std::vector<float2> gv((size_t)nc);
     ERP_HIP_TRY(hipMemcpyAsync(gv.data(), dg.p, sizeof(float2) * gv.size(), hipMemcpyDeviceToHost, L.s));
     ERP_HIP_TRY(hipStreamSynchronize(L.s));

@@ -58,14 +58,22 @@ __device__ __forceinline__ void add_q(int64_t *q, int64_t v) {
 // adds its contributions with global atomics directly.  Global 64-bit atomics
 // are the expensive resource here: they are coherent device-wide and far
 // slower than LDS atomics.
-__global__ void __launch_bounds__(kBlock) k_scatter(
-    const float4 *__restrict__ rec, const float *__restrict__ rratio, int64_t num_cells,
-    int m, int n, float bw, float bh, float inv_bw, float inv_bh, double scale,
-    const int32_t *__restrict__ perm_src, int32_t *__restrict__ perm_dst,
-    int64_t *__restrict__ q) {
+__global__ void __launch_bounds__(kBlock) k_scatter(const JobDesc *__restrict__ jd, int copy_perm) {
     __shared__ unsigned long long tile[kScatterTile];
     __shared__ int wbox[4][kBlock / 64];
+    const SortBufs &sb = jd[blockIdx.y].sb;
+    const int64_t num_cells = sb.num_cells;
     const int64_t base = (int64_t)blockIdx.x * kScatterCells;
+    if (base >= num_cells) return;                      // this job needs fewer workgroups
+    const float4 *__restrict__ rec = sb.rec;
+    const float *__restrict__ rratio = sb.rratio;
+    const int32_t *__restrict__ perm_src = copy_perm ? sb.perm_out : nullptr;
+    int32_t *__restrict__ perm_dst = copy_perm ? sb.perm : nullptr;
+    int64_t *__restrict__ q = jd[blockIdx.y].pb.q_mov;
+    const int m = sb.m, n = sb.n;
+    const float bw = sb.bw, bh = sb.bh;
+    const float inv_bw = 1.f / bw, inv_bh = 1.f / bh;
+    const double scale = jd[blockIdx.y].pb.scale;
     float xi[kScatterPer], yi[kScatterPer], sx[kScatterPer], sy[kScatterPer], r[kScatterPer];
     int bxl[kScatterPer], bxh[kScatterPer], byl[kScatterPer], byh[kScatterPer];
     int lx = INT_MAX, hx = INT_MIN, ly = INT_MAX, hy = INT_MIN;
@@ -168,14 +176,19 @@ __device__ __forceinline__ int sort_key(float lx, float ly, float inv_bw, float 
     return (int)((spread_bits((unsigned)(bx >> shift)) << 1) | spread_bits((unsigned)(by >> shift)));
 }
 
-__global__ void __launch_bounds__(kBlock) k_sort_keys(const int32_t *__restrict__ perm,
-                                                      const float2 *__restrict__ pos,
-                                                      const float2 *__restrict__ coff,
-                                                      int64_t num_cells, float inv_bw,
-                                                      float inv_bh, int m, int n, int shift,
-                                                      int ntiles, int32_t *__restrict__ keys,
-                                                      int32_t *__restrict__ hist) {
+__global__ void __launch_bounds__(kBlock) k_sort_keys(const JobDesc *__restrict__ jd, int shift,
+                                                      int ntiles) {
     __shared__ int lh[kSortTiles];
+    const SortBufs &sb = jd[blockIdx.y].sb;
+    const int64_t num_cells = sb.num_cells;
+    if ((int64_t)blockIdx.x * kScatterCells >= num_cells) return;
+    const int32_t *__restrict__ perm = sb.perm;
+    const float2 *__restrict__ pos = sb.pos;
+    const float2 *__restrict__ coff = sb.coff;
+    int32_t *__restrict__ keys = sb.keys;
+    int32_t *__restrict__ hist = sb.hist;
+    const int m = sb.m, n = sb.n;
+    const float inv_bw = 1.f / sb.bw, inv_bh = 1.f / sb.bh;
     for (int t = threadIdx.x; t < ntiles; t += kBlock) lh[t] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kScatterCells;
@@ -204,9 +217,10 @@ __global__ void __launch_bounds__(kBlock) k_sort_keys(const int32_t *__restrict_
         if (lh[t]) atomicAdd(&hist[t], lh[t]);
 }
 
-__global__ void __launch_bounds__(1024) k_sort_scan(int32_t *__restrict__ hist,
-                                                    int32_t *__restrict__ cursor, int ntiles) {
+__global__ void __launch_bounds__(1024) k_sort_scan(const JobDesc *__restrict__ jd, int ntiles) {
     __shared__ int part[1024];
+    int32_t *__restrict__ hist = jd[blockIdx.x].sb.hist;
+    int32_t *__restrict__ cursor = jd[blockIdx.x].sb.cursor;
     const int per = (ntiles + 1023) / 1024;
     const int t0 = threadIdx.x * per;
     int sum = 0;
@@ -230,18 +244,21 @@ __global__ void __launch_bounds__(1024) k_sort_scan(int32_t *__restrict__ hist,
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_sort_place(const int32_t *__restrict__ perm,
-                                                       const int32_t *__restrict__ keys,
-                                                       const float2 *__restrict__ pos,
-                                                       const float2 *__restrict__ csize,
-                                                       const float2 *__restrict__ coff,
-                                                       const float *__restrict__ ratio,
-                                                       int64_t num_cells, int ntiles,
-                                                       int32_t *__restrict__ cursor,
-                                                       int32_t *__restrict__ perm_out,
-                                                       float4 *__restrict__ rec,
-                                                       float *__restrict__ rratio) {
+__global__ void __launch_bounds__(kBlock) k_sort_place(const JobDesc *__restrict__ jd, int ntiles) {
     __shared__ int lh[kSortTiles];
+    const SortBufs &sb = jd[blockIdx.y].sb;
+    const int64_t num_cells = sb.num_cells;
+    if ((int64_t)blockIdx.x * kScatterCells >= num_cells) return;
+    const int32_t *__restrict__ perm = sb.perm;
+    const int32_t *__restrict__ keys = sb.keys;
+    const float2 *__restrict__ pos = sb.pos;
+    const float2 *__restrict__ csize = sb.csize;
+    const float2 *__restrict__ coff = sb.coff;
+    const float *__restrict__ ratio = sb.ratio;
+    int32_t *__restrict__ cursor = sb.cursor;
+    int32_t *__restrict__ perm_out = sb.perm_out;
+    float4 *__restrict__ rec = sb.rec;
+    float *__restrict__ rratio = sb.rratio;
     for (int t = threadIdx.x; t < ntiles; t += kBlock) lh[t] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kScatterCells;
@@ -346,14 +363,10 @@ inline unsigned grid_for(int64_t count) { return (unsigned)((count + kBlock - 1)
 
 }  // namespace
 
-int launch_scatter(const float4 *rec, const float *rratio, int64_t num_cells, int m, int n,
-                   float bw, float bh, double scale, const int32_t *perm_src, int32_t *perm_dst,
-                   int64_t *q, hipStream_t s) {
-    if (num_cells <= 0) return ERP_OK;
-    const float inv_bw = 1.f / bw, inv_bh = 1.f / bh;
-    const unsigned nb = (unsigned)((num_cells + kScatterCells - 1) / kScatterCells);
-    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, s, rec, rratio, num_cells, m, n, bw,
-                       bh, inv_bw, inv_bh, scale, perm_src, perm_dst, q);
+int launch_scatter(const JobDesc *d, const BatchDims &bd, int nj, bool copy_perm, hipStream_t s) {
+    if (bd.max_cells <= 0) return ERP_OK;
+    const unsigned nb = (unsigned)((bd.max_cells + kScatterCells - 1) / kScatterCells);
+    hipLaunchKernelGGL(k_scatter, dim3(nb, nj), dim3(kBlock), 0, s, d, copy_perm ? 1 : 0);
     ERP_HIP_TRY(hipGetLastError());
     return ERP_OK;
 }
@@ -372,24 +385,20 @@ SortGeom sort_geometry(int m, int n) {
     return g;
 }
 
-int launch_sort(const SortBufs &b, hipStream_t s, KernelTimer *t) {
-    if (b.num_cells <= 0) return ERP_OK;
-    const SortGeom g = sort_geometry(b.m, b.n);
-    const unsigned nb = (unsigned)((b.num_cells + kScatterCells - 1) / kScatterCells);
-    const float inv_bw = 1.f / b.bw, inv_bh = 1.f / b.bh;
+int launch_sort(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s, KernelTimer *t) {
+    if (bd.max_cells <= 0) return ERP_OK;
+    const SortGeom g = sort_geometry(bd.m, bd.n);
+    const unsigned nb = (unsigned)((bd.max_cells + kScatterCells - 1) / kScatterCells);
     if (t) t->begin("sort_keys");
-    hipLaunchKernelGGL(k_sort_keys, dim3(nb), dim3(kBlock), 0, s, b.perm, b.pos, b.coff,
-                       b.num_cells, inv_bw, inv_bh, b.m, b.n, g.shift, g.ntiles, b.keys, b.hist);
+    hipLaunchKernelGGL(k_sort_keys, dim3(nb, nj), dim3(kBlock), 0, s, d, g.shift, g.ntiles);
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
     if (t) t->begin("sort_scan");
-    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, b.hist, b.cursor, g.ntiles);
+    hipLaunchKernelGGL(k_sort_scan, dim3(nj), dim3(1024), 0, s, d, g.ntiles);
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
     if (t) t->begin("sort_place");
-    hipLaunchKernelGGL(k_sort_place, dim3(nb), dim3(kBlock), 0, s, b.perm, b.keys, b.pos, b.csize,
-                       b.coff, b.ratio, b.num_cells, g.ntiles, b.cursor, b.perm_out, b.rec,
-                       b.rratio);
+    hipLaunchKernelGGL(k_sort_place, dim3(nb, nj), dim3(kBlock), 0, s, d, g.ntiles);
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
     return ERP_OK;

@@ -116,10 +116,10 @@ struct PotentialBufs {
     float inv_bin_area;     // T::one() / bin_area in f32
     float bw, bh;
     bool keep_q;            // leave q_mov intact (parity inspection)
+    double scale;           // S of the fixed-point accumulators
+    const float2 *tm_fft, *tm_dct, *tn_fft, *tn_dct;   // spectral tables (length m / n)
 };
 int potential_partials(int m);   // number of partials pass 1 / pass 3 write
-int launch_potential_fft(const PotentialBufs &b, const SpecTables &tm, const SpecTables &tn,
-                         hipStream_t s, KernelTimer *t = nullptr);
 // Potential with MFMA dense transforms (erp_mfma.hip).
 struct MfmaPlan;
 int mfma_plan_create(MfmaPlan **plan, int m, int n, hipStream_t s);

@@ -11,11 +11,6 @@ constexpr int kScatterCells = kScatterPer * kBlock;   // cells per workgroup
 constexpr int kScatterTile = 4096;                    // LDS int64 bins per workgroup
 constexpr int kScatterSide = 64;                      // tile side when the box is too big
 constexpr int kSortTiles = 4096;                      // max sort tiles (64 x 64 Morton)
-// rec = (x + ox, y + oy, cw, ch) and rratio per cell, in processing order;
-// perm_src -> perm_dst (both nullable) is copied on the way.
-int launch_scatter(const float4 *rec, const float *rratio, int64_t num_cells, int m, int n,
-                   float bw, float bh, double scale, const int32_t *perm_src, int32_t *perm_dst,
-                   int64_t *q, hipStream_t s);
 struct SortGeom {
     int shift, ntiles;
 };
@@ -32,7 +27,6 @@ struct SortBufs {
     float4 *rec;         // [C] sorted scatter records
     float *rratio;       // [C]
 };
-int launch_sort(const SortBufs &b, hipStream_t s, KernelTimer *t = nullptr);
 int launch_pack_records(const float2 *pos, const float2 *csize, const float2 *coff,
                         int64_t num_cells, float4 *rec, hipStream_t s);
 int launch_iota(int32_t *v, int64_t count, hipStream_t s);
@@ -53,15 +47,16 @@ struct WaNets {
     const int32_t *net_start;   // [E+1]
     int64_t n2;                 // nets [0, n2) have two pins, slots 2e, 2e+1
     int64_t bucket_begin[kWaBuckets + 1];  // multi-pin nets, sorted by degree, per bucket
+    int32_t part_offset[kWaBuckets + 2];   // first partial of the 2-pin launch / bucket b (b+1)
+    const float2 *pos;          // [N] instance positions
+    const double *gamma;        // WA gamma (the scheduler's, or an explicit one)
     float2 *slot_grad;          // [P] output
     double *hpwl_part;          // partial sums, one per workgroup (see wa_partials)
 };
-int wa_partials(const WaNets &w);
-int launch_wa(const WaNets &w, const float2 *pos, const double *gamma_dev, hipStream_t s,
-              KernelTimer *t = nullptr);
+// sets part_offset and returns the number of partials
+int wa_partials(WaNets &w);
 // Exact HPWL in int64 DBU over all nets.
-int launch_hpwl_dbu(const WaNets &w, const float2 *pos, int64_t num_nets, unsigned long long *out,
-                    hipStream_t s);
+int launch_hpwl_dbu(const WaNets &w, int64_t num_nets, unsigned long long *out, hipStream_t s);
 
 // ------------------------------------------------------------ optimizer
 struct CellBufs {
@@ -80,20 +75,9 @@ struct CellBufs {
     const float *fx, *fy;       // field maps
     int m, n;
     float bw, bh;
+    DevState *st;               // scalar state (density weight, alpha, beta)
 };
 int cell_partials(int64_t C);
-// pin -> cell gather of the WA gradient (hpwl.rs:113-135)
-int launch_pin2cell(const CellBufs &c, hipStream_t s);
-// field gather (eforce.rs:15-91)
-int launch_efield(const CellBufs &c, hipStream_t s);
-// L1 norms for the first iteration's density-weight scaling (nesterov.rs:47-55)
-int launch_l1(const CellBufs &c, hipStream_t s);
-// combined preconditioned gradient + BB dot-product partials (nesterov.rs:74-95)
-int launch_combine(const CellBufs &c, const DevState *st, hipStream_t s);
-// fused per-cell pass: pin->cell gather, field gather, combine (iterations > 0)
-int launch_cell_fused(const CellBufs &c, const DevState *st, bool store_parts, hipStream_t s);
-// look-ahead step with die clamp and history (nesterov.rs:111-145)
-int launch_update(const CellBufs &c, const DevState *st, hipStream_t s);
 
 // Scalar work on the device state (one workgroup).
 enum ScalarOps {
@@ -111,7 +95,6 @@ struct ScalarArgs {
     const double *part_a, *part_b; int n_cell;
     double inv_scale_over_area;   // 1 / (S * movable_area)
 };
-int launch_scalar(const ScalarArgs &a, int ops, hipStream_t s);
 int launch_reset_state(DevState *st, double gamma0, double dw0, hipStream_t s);
 int launch_init_positions(float2 *pos, const float *w, const float *h, const float *x,
                           const float *y, int64_t num_nodes, int64_t num_movable, float die_w,
@@ -120,5 +103,44 @@ int launch_setup_cells(const float *w, const float *h, const float *npins, int64
                        int64_t num_movable, float bw, float bh, float target_density,
                        float die_w, float die_h, float2 *csize, float2 *coff, float *ratio,
                        float *area, float *npins_out, float2 *hi, hipStream_t s);
+
+// ------------------------------------------------------------ batched execution
+// Everything one placement's per-iteration kernels read, resident in device
+// memory.  Every per-iteration kernel takes an array of these and serves job
+// blockIdx.y (or blockIdx.x for one-workgroup kernels), so jobs that share a bin
+// grid run as one launch per kernel; a single placement is a batch of one.
+struct JobDesc {
+    WaNets wa;
+    SortBufs sb;
+    PotentialBufs pb;
+    CellBufs cb;
+    ScalarArgs sa;
+};
+
+// Launch geometry of a batch: the largest per-job work of each kernel.
+struct BatchDims {
+    int m = 0, n = 0;                   // common bin grid
+    int64_t max_cells = 0;              // movable cells
+    int64_t max_n2 = 0;                 // two-pin nets
+    int64_t max_bucket[kWaBuckets] = {};
+    void add(const JobDesc &j);         // widen to include job j
+};
+
+int launch_wa(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
+              KernelTimer *t = nullptr);
+int launch_sort(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
+                KernelTimer *t = nullptr);
+// perm_out -> perm is copied by the scatter when copy_perm is set
+int launch_scatter(const JobDesc *d, const BatchDims &bd, int nj, bool copy_perm, hipStream_t s);
+int launch_pin2cell(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
+int launch_efield(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
+int launch_l1(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
+int launch_combine(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
+int launch_cell_fused(const JobDesc *d, const BatchDims &bd, int nj, bool store_parts,
+                      hipStream_t s);
+int launch_update(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
+int launch_scalar(const JobDesc *d, int nj, int ops, hipStream_t s);
+int launch_potential_fft(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
+                         KernelTimer *t = nullptr);
 
 }  // namespace erp

@@ -96,19 +96,24 @@ __device__ __forceinline__ float2 combine(const CellBufs &c, int64_t i, float2 g
     return make_float2((gwl.x + dw * gd.x) * inv_p, (gwl.y + dw * gd.y) * inv_p);
 }
 
-__global__ void __launch_bounds__(kBlock) k_pin2cell(CellBufs c) {
+#define ERP_CELL_JOB()                                                              \
+    const CellBufs &c = jd[blockIdx.y].cb;                                         \
+    if ((int64_t)blockIdx.x * kBlock >= c.C) return; /* uniform per workgroup */   \
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+
+__global__ void __launch_bounds__(kBlock) k_pin2cell(const JobDesc *__restrict__ jd) {
+    ERP_CELL_JOB();
     if (i < c.C) c.gwl[i] = pin2cell(c, i);
 }
 
-__global__ void __launch_bounds__(kBlock) k_efield(CellBufs c) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kBlock) k_efield(const JobDesc *__restrict__ jd) {
+    ERP_CELL_JOB();
     if (i < c.C) c.gd[i] = efield_cell(c, i, 1.f / c.bw, 1.f / c.bh);
 }
 
-__global__ void __launch_bounds__(kBlock) k_l1(CellBufs c) {
+__global__ void __launch_bounds__(kBlock) k_l1(const JobDesc *__restrict__ jd) {
     __shared__ double red[kBlock];
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    ERP_CELL_JOB();
     double a = 0.0, b = 0.0;
     if (i < c.C) {
         const float2 w = c.gwl[i], d = c.gd[i];
@@ -133,9 +138,10 @@ __device__ __forceinline__ void bb_terms(const CellBufs &c, int64_t i, float2 g,
     b = gx * gx + gy * gy;
 }
 
-__global__ void __launch_bounds__(kBlock) k_combine(CellBufs c, const DevState *__restrict__ st) {
+__global__ void __launch_bounds__(kBlock) k_combine(const JobDesc *__restrict__ jd) {
     __shared__ double red[kBlock];
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    ERP_CELL_JOB();
+    const DevState *__restrict__ st = c.st;
     double a = 0.0, b = 0.0;
     if (i < c.C) {
         const float2 g = combine(c, i, c.gwl[i], c.gd[i], (float)st->density_weight);
@@ -151,9 +157,10 @@ __global__ void __launch_bounds__(kBlock) k_combine(CellBufs c, const DevState *
 }
 
 template <bool STORE_PARTS>
-__global__ void __launch_bounds__(kBlock) k_cell_fused(CellBufs c, const DevState *__restrict__ st) {
+__global__ void __launch_bounds__(kBlock) k_cell_fused(const JobDesc *__restrict__ jd) {
     __shared__ double red[kBlock];
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    ERP_CELL_JOB();
+    const DevState *__restrict__ st = c.st;
     double a = 0.0, b = 0.0;
     if (i < c.C) {
         const float2 gwl = pin2cell(c, i);
@@ -175,8 +182,9 @@ __global__ void __launch_bounds__(kBlock) k_cell_fused(CellBufs c, const DevStat
 }
 
 // nesterov.rs:111-145
-__global__ void __launch_bounds__(kBlock) k_update(CellBufs c, const DevState *__restrict__ st) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kBlock) k_update(const JobDesc *__restrict__ jd) {
+    ERP_CELL_JOB();
+    const DevState *__restrict__ st = c.st;
     if (i >= c.C) return;
     const float alpha = (float)st->alpha, beta = (float)st->beta;
     const float2 p = c.pos[i], pp = c.prev_pos[i], g = c.g[i], hi = c.hi[i];
@@ -195,23 +203,26 @@ __global__ void __launch_bounds__(kBlock) k_update(CellBufs c, const DevState *_
 // Deterministic sums of the partial arrays by one workgroup: each thread adds a
 // fixed strided subset with four independent accumulators (so its loads are in
 // flight together), then a fixed tree combines the threads.
+constexpr int kScalarThreads = 1024;
+
 template <class T>
 __device__ __forceinline__ T strided_sum(const T *part, int count) {
     T v0 = 0, v1 = 0, v2 = 0, v3 = 0;
     int k = threadIdx.x;
-    for (; k + 3 * kBlock < count; k += 4 * kBlock) {
+    for (; k + 3 * kScalarThreads < count; k += 4 * kScalarThreads) {
         v0 += part[k];
-        v1 += part[k + kBlock];
-        v2 += part[k + 2 * kBlock];
-        v3 += part[k + 3 * kBlock];
+        v1 += part[k + kScalarThreads];
+        v2 += part[k + 2 * kScalarThreads];
+        v3 += part[k + 3 * kScalarThreads];
     }
-    for (; k < count; k += kBlock) v0 += part[k];
+    for (; k < count; k += kScalarThreads) v0 += part[k];
     return (v0 + v1) + (v2 + v3);
 }
 
-__global__ void __launch_bounds__(kBlock) k_scalar(ScalarArgs a, int ops) {
-    __shared__ double redd[4][kBlock];
-    __shared__ int64_t redi[kBlock];
+__global__ void __launch_bounds__(kScalarThreads) k_scalar(const JobDesc *__restrict__ jd, int ops) {
+    const ScalarArgs &a = jd[blockIdx.x].sa;
+    __shared__ double redd[4][kScalarThreads];
+    __shared__ int64_t redi[kScalarThreads];
     double h = 0, e = 0, pa = 0, pb = 0;
     int64_t o = 0;
     if (ops & SC_HPWL) h = strided_sum(a.hpwl_part, a.n_hpwl);
@@ -229,7 +240,7 @@ __global__ void __launch_bounds__(kBlock) k_scalar(ScalarArgs a, int ops) {
     redd[3][threadIdx.x] = pb;
     redi[threadIdx.x] = o;
     __syncthreads();
-    for (int st = kBlock / 2; st > 0; st >>= 1) {
+    for (int st = kScalarThreads / 2; st > 0; st >>= 1) {
         if ((int)threadIdx.x < st) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) redd[q][threadIdx.x] += redd[q][threadIdx.x + st];
@@ -343,30 +354,37 @@ inline unsigned grid_for(int64_t count) { return (unsigned)((count + kBlock - 1)
 
 int cell_partials(int64_t C) { return (int)grid_for(C > 0 ? C : 1); }
 
-#define ERP_LAUNCH_CELLS(kern, ...)                                                     \
-    do {                                                                                \
-        if (c.C <= 0) return ERP_OK;                                                    \
-        hipLaunchKernelGGL(kern, dim3(grid_for(c.C)), dim3(kBlock), 0, s, __VA_ARGS__); \
-        ERP_HIP_TRY(hipGetLastError());                                                 \
-        return ERP_OK;                                                                  \
+inline dim3 cell_grid(const BatchDims &bd, int nj) { return dim3(grid_for(bd.max_cells), nj); }
+
+#define ERP_LAUNCH_CELLS(kern, ...)                                                       \
+    do {                                                                                  \
+        if (bd.max_cells <= 0) return ERP_OK;                                             \
+        hipLaunchKernelGGL(kern, cell_grid(bd, nj), dim3(kBlock), 0, s, __VA_ARGS__);     \
+        ERP_HIP_TRY(hipGetLastError());                                                   \
+        return ERP_OK;                                                                    \
     } while (0)
 
-int launch_pin2cell(const CellBufs &c, hipStream_t s) { ERP_LAUNCH_CELLS(k_pin2cell, c); }
-int launch_efield(const CellBufs &c, hipStream_t s) { ERP_LAUNCH_CELLS(k_efield, c); }
-int launch_l1(const CellBufs &c, hipStream_t s) { ERP_LAUNCH_CELLS(k_l1, c); }
-int launch_combine(const CellBufs &c, const DevState *st, hipStream_t s) {
-    ERP_LAUNCH_CELLS(k_combine, c, st);
+int launch_pin2cell(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s) {
+    ERP_LAUNCH_CELLS(k_pin2cell, d);
 }
-int launch_cell_fused(const CellBufs &c, const DevState *st, bool store_parts, hipStream_t s) {
-    if (store_parts) ERP_LAUNCH_CELLS(k_cell_fused<true>, c, st);
-    ERP_LAUNCH_CELLS(k_cell_fused<false>, c, st);
+int launch_efield(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s) {
+    ERP_LAUNCH_CELLS(k_efield, d);
 }
-int launch_update(const CellBufs &c, const DevState *st, hipStream_t s) {
-    ERP_LAUNCH_CELLS(k_update, c, st);
+int launch_l1(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s) { ERP_LAUNCH_CELLS(k_l1, d); }
+int launch_combine(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s) {
+    ERP_LAUNCH_CELLS(k_combine, d);
+}
+int launch_cell_fused(const JobDesc *d, const BatchDims &bd, int nj, bool store_parts,
+                      hipStream_t s) {
+    if (store_parts) ERP_LAUNCH_CELLS(k_cell_fused<true>, d);
+    ERP_LAUNCH_CELLS(k_cell_fused<false>, d);
+}
+int launch_update(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s) {
+    ERP_LAUNCH_CELLS(k_update, d);
 }
 
-int launch_scalar(const ScalarArgs &a, int ops, hipStream_t s) {
-    hipLaunchKernelGGL(k_scalar, dim3(1), dim3(kBlock), 0, s, a, ops);
+int launch_scalar(const JobDesc *d, int nj, int ops, hipStream_t s) {
+    hipLaunchKernelGGL(k_scalar, dim3(nj), dim3(kScalarThreads), 0, s, d, ops);
     ERP_HIP_TRY(hipGetLastError());
     return ERP_OK;
 }

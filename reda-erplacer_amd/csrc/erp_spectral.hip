@@ -22,6 +22,7 @@
 #include <mutex>
 
 #include "erp_internal.h"
+#include "erp_kernels.h"
 #include "erp_timer.h"
 
 namespace erp {
@@ -132,7 +133,7 @@ __device__ __forceinline__ float2 spec_pair(const float *Y, int k, int L, int st
 // R = 2F rows per block.  SRC_FIXED: rows come from the fixed-point accumulators
 // (pass 1); otherwise from a float map.  Output T = unscaled row DCT sums.
 template <bool SRC_FIXED>
-__global__ void __launch_bounds__(kBlock) k_row_dct(
+__device__ __forceinline__ void row_dct_body(
     const float *__restrict__ x, int64_t *__restrict__ q_mov, const int64_t *__restrict__ q_fix,
     float *__restrict__ rho_out, int64_t *__restrict__ ovf_part, float *__restrict__ T, int n,
     int logN, int F, const float2 *__restrict__ fft_tw, const float2 *__restrict__ dct_tw,
@@ -183,6 +184,22 @@ __global__ void __launch_bounds__(kBlock) k_row_dct(
         T[o0] = 0.5f * y.x;
         T[o0 + n] = 0.5f * y.y;
     }
+}
+
+// leaf: rows of a float map
+__global__ void __launch_bounds__(kBlock) k_row_dct(const float *__restrict__ x,
+                                                    float *__restrict__ T, int n, int logN, int F,
+                                                    const float2 *__restrict__ fft_tw,
+                                                    const float2 *__restrict__ dct_tw) {
+    row_dct_body<false>(x, nullptr, nullptr, nullptr, nullptr, T, n, logN, F, fft_tw, dct_tw, 0.0,
+                        0, 0.f, false);
+}
+
+// pass 1 of the potential pipeline for job blockIdx.y
+__global__ void __launch_bounds__(kBlock) k_pass1(const JobDesc *__restrict__ jd, int logN, int F) {
+    const PotentialBufs &b = jd[blockIdx.y].pb;
+    row_dct_body<true>(nullptr, b.q_mov, b.q_fix, b.rho, b.ovf_part, b.rowt, b.n, logN, F,
+                       b.tn_fft, b.tn_dct, b.inv_scale, b.coef_q, b.inv_bin_area, !b.keep_q);
 }
 
 // -------------------------------------------------------------- column DCT-II
@@ -290,12 +307,20 @@ __global__ void __launch_bounds__(kBlock) k_row_inv(const float *__restrict__ X,
 // spectral products (frequencies.rs:4-72 evaluated in f32 on the fly, in the
 // reference's operation order); three column inverses:
 //   P1 = IDXST_x(a Wx), P2 = IDCT_x(a Wy), P3 = IDCT_x(a inv).
-__global__ void __launch_bounds__(kBlock) k_col_spectral(
-    const float *__restrict__ T, float *__restrict__ P1, float *__restrict__ P2,
-    float *__restrict__ P3, float *__restrict__ auv_out, int m, int logM, int n, int G,
-    const float2 *__restrict__ fft_tw, const float2 *__restrict__ dct_tw, float scale,
-    float two_pi_over_m, float two_pi_over_n, float wv_div) {
+__global__ void __launch_bounds__(kBlock) k_col_spectral(const JobDesc *__restrict__ jd, int logM,
+                                                         int G, float scale, float two_pi_over_m,
+                                                         float two_pi_over_n) {
     extern __shared__ float2 lds[];
+    const PotentialBufs &pb = jd[blockIdx.y].pb;
+    const float *__restrict__ T = pb.rowt;
+    float *__restrict__ P1 = pb.p1;
+    float *__restrict__ P2 = pb.p2;
+    float *__restrict__ P3 = pb.p3;
+    float *__restrict__ auv_out = pb.auv;
+    const int m = pb.m, n = pb.n;
+    const float2 *__restrict__ fft_tw = pb.tm_fft;
+    const float2 *__restrict__ dct_tw = pb.tm_dct;
+    const float wv_div = pb.bw / pb.bh;
     const int S = 2 * G;
     float2 *tw = lds;
     float2 *buf = lds + (m >> 1);                          // 3G complex arrays of m
@@ -368,12 +393,20 @@ __global__ void __launch_bounds__(kBlock) k_col_spectral(
 // ----------------------------------------------------- fused row pass (pass 3)
 // field_x = IDCT_y(P1), field_y = IDXST_y(P2), potential = IDCT_y(P3);
 // energy partial = sum potential * density (epotential.rs:344-346).
-__global__ void __launch_bounds__(kBlock) k_row_final(
-    const float *__restrict__ P1, const float *__restrict__ P2, const float *__restrict__ P3,
-    const float *__restrict__ rho, float *__restrict__ fx, float *__restrict__ fy,
-    float *__restrict__ pot, double *__restrict__ energy_part, int n, int logN, int F,
-    const float2 *__restrict__ fft_tw, const float2 *__restrict__ dct_tw) {
+__global__ void __launch_bounds__(kBlock) k_row_final(const JobDesc *__restrict__ jd, int logN, int F) {
     extern __shared__ float2 lds[];
+    const PotentialBufs &pb = jd[blockIdx.y].pb;
+    const float *__restrict__ P1 = pb.p1;
+    const float *__restrict__ P2 = pb.p2;
+    const float *__restrict__ P3 = pb.p3;
+    const float *__restrict__ rho = pb.rho;
+    float *__restrict__ fx = pb.fx;
+    float *__restrict__ fy = pb.fy;
+    float *__restrict__ pot = pb.pot;
+    double *__restrict__ energy_part = pb.energy_part;
+    const int n = pb.n;
+    const float2 *__restrict__ fft_tw = pb.tn_fft;
+    const float2 *__restrict__ dct_tw = pb.tn_dct;
     float2 *tw = lds;
     float2 *buf = lds + (n >> 1);   // 3F arrays
     load_tw(tw, fft_tw, n >> 1);
@@ -473,11 +506,10 @@ int launch_dct2(const float *x, float *out, float *scratch, int m, int n, const 
                 const SpecTables &tn, hipStream_t s) {
     const int F = kRowPairs, logN = ilog2(n), logM = ilog2(m), G = col_pairs(m);
     size_t lr = row_lds(n, F);
-    int rc = set_lds_limit((const void *)k_row_dct<false>, lr);
+    int rc = set_lds_limit((const void *)k_row_dct, lr);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_row_dct<false>, dim3(m / (2 * F)), dim3(kBlock), lr, s, x, nullptr,
-                       nullptr, nullptr, nullptr, scratch, n, logN, F, tn.fft_tw.as<float2>(),
-                       tn.dct_tw.as<float2>(), 0.0, (int64_t)0, 0.f, false);
+    hipLaunchKernelGGL(k_row_dct, dim3(m / (2 * F)), dim3(kBlock), lr, s, x, scratch, n, logN, F,
+                       tn.fft_tw.as<float2>(), tn.dct_tw.as<float2>());
     ERP_HIP_TRY(hipGetLastError());
     size_t lc = row_lds(m, G);
     if ((rc = set_lds_limit((const void *)k_col_dct, lc))) return rc;
@@ -517,40 +549,32 @@ int launch_inverse2(const float *x, float *out, float *scratch, int m, int n, In
     return ERP_OK;
 }
 
-int launch_potential_fft(const PotentialBufs &b, const SpecTables &tm, const SpecTables &tn,
-                         hipStream_t s, KernelTimer *t) {
-    const int m = b.m, n = b.n, F = kRowPairs, logN = ilog2(n), logM = ilog2(m), G = col_pairs(m);
+int launch_potential_fft(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
+                         KernelTimer *t) {
+    const int m = bd.m, n = bd.n, F = kRowPairs, logN = ilog2(n), logM = ilog2(m), G = col_pairs(m);
     // pass 1
     size_t l1 = row_lds(n, F);
-    int rc = set_lds_limit((const void *)k_row_dct<true>, l1);
+    int rc = set_lds_limit((const void *)k_pass1, l1);
     if (rc) return rc;
     if (t) t->begin("pass1_row_dct");
-    hipLaunchKernelGGL(k_row_dct<true>, dim3(m / (2 * F)), dim3(kBlock), l1, s, nullptr, b.q_mov,
-                       b.q_fix, b.rho, b.ovf_part, b.rowt, n, logN, F, tn.fft_tw.as<float2>(),
-                       tn.dct_tw.as<float2>(), b.inv_scale, b.coef_q, b.inv_bin_area,
-                       !b.keep_q);
+    hipLaunchKernelGGL(k_pass1, dim3(m / (2 * F), nj), dim3(kBlock), l1, s, d, logN, F);
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
     // pass 2 (operator constants exactly as frequencies.rs computes them in T=f32)
     const float two_pi_over_m = (float)(2.0 * M_PI) / (float)m;
     const float two_pi_over_n = (float)(2.0 * M_PI) / (float)n;
-    const float wv_div = b.bw / b.bh;
     size_t l2 = sizeof(float2) * ((size_t)m / 2 + (size_t)3 * G * m) + sizeof(float) * (size_t)2 * G * m;
     if ((rc = set_lds_limit((const void *)k_col_spectral, l2))) return rc;
     if (t) t->begin("pass2_col_spectral");
-    hipLaunchKernelGGL(k_col_spectral, dim3(n / (2 * G)), dim3(kBlock), l2, s, b.rowt, b.p1, b.p2,
-                       b.p3, b.auv, m, logM, n, G, tm.fft_tw.as<float2>(), tm.dct_tw.as<float2>(),
-                       (float)(4.0 / ((double)m * (double)n)), two_pi_over_m, two_pi_over_n,
-                       wv_div);
+    hipLaunchKernelGGL(k_col_spectral, dim3(n / (2 * G), nj), dim3(kBlock), l2, s, d, logM, G,
+                       (float)(4.0 / ((double)m * (double)n)), two_pi_over_m, two_pi_over_n);
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
     // pass 3
     size_t l3 = row_lds(n, 3 * F);
     if ((rc = set_lds_limit((const void *)k_row_final, l3))) return rc;
     if (t) t->begin("pass3_row_final");
-    hipLaunchKernelGGL(k_row_final, dim3(m / (2 * F)), dim3(kBlock), l3, s, b.p1, b.p2, b.p3,
-                       b.rho, b.fx, b.fy, b.pot, b.energy_part, n, logN, F,
-                       tn.fft_tw.as<float2>(), tn.dct_tw.as<float2>());
+    hipLaunchKernelGGL(k_row_final, dim3(m / (2 * F), nj), dim3(kBlock), l3, s, d, logN, F);
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
     return ERP_OK;

@@ -13,6 +13,7 @@
 //   dx = x - max, e = exp(dx/gamma), S = sum e, D = sum dx e,
 //   dn = x - min, e' = exp(-dn/gamma), S' = sum e', D' = sum dn e'.
 // which avoids the cancellation of a + b x (both ~ x/gamma) in f32.
+#include <algorithm>
 #include <climits>
 
 #include "erp_internal.h"
@@ -47,27 +48,25 @@ __device__ __forceinline__ float2 wa2(float a, float b, float ig) {
     return make_float2(ga, gb);
 }
 
-__global__ void __launch_bounds__(kBlock) k_wa_2pin(const int32_t *__restrict__ slot_node,
-                                                    const float2 *__restrict__ slot_off,
-                                                    const float2 *__restrict__ pos, int64_t n2,
-                                                    const double *__restrict__ gamma,
-                                                    float2 *__restrict__ slot_grad,
-                                                    double *__restrict__ part) {
+__global__ void __launch_bounds__(kBlock) k_wa_2pin(const JobDesc *__restrict__ jd) {
+    const WaNets &w = jd[blockIdx.y].wa;
+    const int64_t n2 = w.n2;
+    if ((int64_t)blockIdx.x * kBlock >= n2) return;    // this job needs fewer workgroups
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double h = 0.0;
     if (e < n2) {
-        const float ig = 1.f / (float)(*gamma);
-        const int2 nd = reinterpret_cast<const int2 *>(slot_node)[e];
-        const float4 of = reinterpret_cast<const float4 *>(slot_off)[e];
-        const float2 p0 = pos[nd.x], p1 = pos[nd.y];
+        const float ig = 1.f / (float)(*w.gamma);
+        const int2 nd = reinterpret_cast<const int2 *>(w.slot_node)[e];
+        const float4 of = reinterpret_cast<const float4 *>(w.slot_off)[e];
+        const float2 p0 = w.pos[nd.x], p1 = w.pos[nd.y];
         const float x0 = p0.x + of.x, y0 = p0.y + of.y;
         const float x1 = p1.x + of.z, y1 = p1.y + of.w;
         h = (double)((fmaxf(x0, x1) - fminf(x0, x1)) + (fmaxf(y0, y1) - fminf(y0, y1)));
         const float2 gx = wa2(x0, x1, ig), gy = wa2(y0, y1, ig);
-        reinterpret_cast<float4 *>(slot_grad)[e] = make_float4(gx.x, gy.x, gx.y, gy.y);
+        reinterpret_cast<float4 *>(w.slot_grad)[e] = make_float4(gx.x, gy.x, gx.y, gy.y);
     }
-    const double s = block_sum_f64(h);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    const double sum = block_sum_f64(h);
+    if (threadIdx.x == 0) w.hpwl_part[w.part_offset[0] + blockIdx.x] = sum;
 }
 
 // One net per group of G lanes (G divides 64), each lane holding up to K of the
@@ -75,14 +74,16 @@ __global__ void __launch_bounds__(kBlock) k_wa_2pin(const int32_t *__restrict__ 
 // three passes of hpwl.rs:248-334 run on registers, and the max/min and the
 // exponential sums are reduced with shuffles inside the group.
 template <int G, int K>
-__global__ void __launch_bounds__(kBlock) k_wa_group(const int32_t *__restrict__ slot_node,
-                                                     const float2 *__restrict__ slot_off,
-                                                     const int32_t *__restrict__ net_start,
-                                                     const float2 *__restrict__ pos,
-                                                     int64_t e0, int64_t e1,
-                                                     const double *__restrict__ gamma,
-                                                     float2 *__restrict__ slot_grad,
-                                                     double *__restrict__ part) {
+__global__ void __launch_bounds__(kBlock) k_wa_group(const JobDesc *__restrict__ jd, int bucket) {
+    const WaNets &w = jd[blockIdx.y].wa;
+    const int64_t e0 = w.bucket_begin[bucket], e1 = w.bucket_begin[bucket + 1];
+    if ((int64_t)blockIdx.x * kBlock >= (e1 - e0) * G) return;   // uniform per workgroup
+    const int32_t *__restrict__ slot_node = w.slot_node;
+    const float2 *__restrict__ slot_off = w.slot_off;
+    const int32_t *__restrict__ net_start = w.net_start;
+    const float2 *__restrict__ pos = w.pos;
+    float2 *__restrict__ slot_grad = w.slot_grad;
+    const double *gamma = w.gamma;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t e = e0 + t / G;
     const int r = (int)(t & (G - 1));
@@ -155,7 +156,7 @@ __global__ void __launch_bounds__(kBlock) k_wa_group(const int32_t *__restrict__
         }
     }
     const double sum = block_sum_f64(h);
-    if (threadIdx.x == 0) part[blockIdx.x] = sum;
+    if (threadIdx.x == 0) w.hpwl_part[w.part_offset[bucket + 1] + blockIdx.x] = sum;
 }
 
 __device__ __forceinline__ float wave_max(float v) {
@@ -175,14 +176,16 @@ __device__ __forceinline__ float wave_fsum(float v) {
 }
 
 // One net per wavefront (4 per workgroup).
-__global__ void __launch_bounds__(kBlock) k_wa_large(const int32_t *__restrict__ slot_node,
-                                                     const float2 *__restrict__ slot_off,
-                                                     const int32_t *__restrict__ net_start,
-                                                     const float2 *__restrict__ pos,
-                                                     int64_t e0, int64_t e1,
-                                                     const double *__restrict__ gamma,
-                                                     float2 *__restrict__ slot_grad,
-                                                     double *__restrict__ part) {
+__global__ void __launch_bounds__(kBlock) k_wa_large(const JobDesc *__restrict__ jd, int bucket) {
+    const WaNets &w = jd[blockIdx.y].wa;
+    const int64_t e0 = w.bucket_begin[bucket], e1 = w.bucket_begin[bucket + 1];
+    if ((int64_t)blockIdx.x * (kBlock / 64) >= e1 - e0) return;  // uniform per workgroup
+    const int32_t *__restrict__ slot_node = w.slot_node;
+    const float2 *__restrict__ slot_off = w.slot_off;
+    const int32_t *__restrict__ net_start = w.net_start;
+    const float2 *__restrict__ pos = w.pos;
+    float2 *__restrict__ slot_grad = w.slot_grad;
+    const double *gamma = w.gamma;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t e = e0 + (int64_t)blockIdx.x * (kBlock / 64) + wave;
     double h = 0.0;
@@ -223,8 +226,8 @@ __global__ void __launch_bounds__(kBlock) k_wa_large(const int32_t *__restrict__
             slot_grad[k] = make_float2(gx, gy);
         }
     }
-    const double s = block_sum_f64(h);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    const double sum = block_sum_f64(h);
+    if (threadIdx.x == 0) w.hpwl_part[w.part_offset[bucket + 1] + blockIdx.x] = sum;
 }
 
 // Exact HPWL: pin coordinates rounded to integer DBU (nearest-even).
@@ -260,76 +263,72 @@ inline unsigned blocks(int64_t count, int per) { return (unsigned)((count + per 
 const int kWaDegree[kWaBuckets] = {4, 8, 16, 32, 64, 128, 256, 512, 1 << 30};
 
 namespace {
-template <int G, int K>
-int launch_group(const WaNets &w, int b, const float2 *pos, const double *gamma, double *part,
-                 unsigned nb, hipStream_t s) {
-    hipLaunchKernelGGL((k_wa_group<G, K>), dim3(nb), dim3(kBlock), 0, s, w.slot_node, w.slot_off,
-                       w.net_start, pos, w.bucket_begin[b], w.bucket_begin[b + 1], gamma,
-                       w.slot_grad, part);
-    ERP_HIP_TRY(hipGetLastError());
-    return ERP_OK;
-}
 // lanes per net of bucket b (the last bucket loops with one wavefront per net)
 constexpr int kWaLanes[kWaBuckets] = {1, 2, 4, 8, 16, 32, 64, 64, 64};
 const char *const kWaNames[kWaBuckets] = {"wa_deg4", "wa_deg8", "wa_deg16", "wa_deg32",
                                           "wa_deg64", "wa_deg128", "wa_deg256", "wa_deg512",
                                           "wa_large"};
-unsigned bucket_blocks(const WaNets &w, int b) {
-    const int64_t nets = w.bucket_begin[b + 1] - w.bucket_begin[b];
-    return blocks(nets * kWaLanes[b], kBlock);
+unsigned bucket_blocks(int64_t nets, int b) {
+    return b == kWaBuckets - 1 ? blocks(nets, kBlock / 64) : blocks(nets * kWaLanes[b], kBlock);
+}
+template <int G, int K>
+void launch_group(const JobDesc *d, int b, unsigned nb, int nj, hipStream_t s) {
+    hipLaunchKernelGGL((k_wa_group<G, K>), dim3(nb, nj), dim3(kBlock), 0, s, d, b);
 }
 }  // namespace
 
-int wa_partials(const WaNets &w) {
+void BatchDims::add(const JobDesc &j) {
+    m = j.pb.m;
+    n = j.pb.n;
+    max_cells = std::max<int64_t>(max_cells, j.cb.C);
+    max_n2 = std::max<int64_t>(max_n2, j.wa.n2);
+    for (int b = 0; b < kWaBuckets; ++b)
+        max_bucket[b] = std::max<int64_t>(max_bucket[b], j.wa.bucket_begin[b + 1] - j.wa.bucket_begin[b]);
+}
+
+int wa_partials(WaNets &w) {
     int total = (int)blocks(w.n2, kBlock);
-    for (int b = 0; b < kWaBuckets; ++b) total += (int)bucket_blocks(w, b);
+    w.part_offset[0] = 0;
+    for (int b = 0; b < kWaBuckets; ++b) {
+        w.part_offset[b + 1] = total;
+        total += (int)bucket_blocks(w.bucket_begin[b + 1] - w.bucket_begin[b], b);
+    }
     return total;
 }
 
-int launch_wa(const WaNets &w, const float2 *pos, const double *gamma_dev, hipStream_t s,
-              KernelTimer *t) {
-    double *part = w.hpwl_part;
-    const unsigned b2 = blocks(w.n2, kBlock);
+int launch_wa(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s, KernelTimer *t) {
+    const unsigned b2 = blocks(bd.max_n2, kBlock);
     if (b2) {
         if (t) t->begin("wa_2pin");
-        hipLaunchKernelGGL(k_wa_2pin, dim3(b2), dim3(kBlock), 0, s, w.slot_node, w.slot_off, pos,
-                           w.n2, gamma_dev, w.slot_grad, part);
+        hipLaunchKernelGGL(k_wa_2pin, dim3(b2, nj), dim3(kBlock), 0, s, d);
         if (t) t->end();
         ERP_HIP_TRY(hipGetLastError());
     }
-    part += b2;
     for (int b = 0; b < kWaBuckets; ++b) {
-        const unsigned nb = bucket_blocks(w, b);
+        const unsigned nb = bucket_blocks(bd.max_bucket[b], b);
         if (!nb) continue;
         if (t) t->begin(kWaNames[b]);
-        int rc;
         switch (b) {
-        case 0: rc = launch_group<1, 4>(w, b, pos, gamma_dev, part, nb, s); break;
-        case 1: rc = launch_group<2, 4>(w, b, pos, gamma_dev, part, nb, s); break;
-        case 2: rc = launch_group<4, 4>(w, b, pos, gamma_dev, part, nb, s); break;
-        case 3: rc = launch_group<8, 4>(w, b, pos, gamma_dev, part, nb, s); break;
-        case 4: rc = launch_group<16, 4>(w, b, pos, gamma_dev, part, nb, s); break;
-        case 5: rc = launch_group<32, 4>(w, b, pos, gamma_dev, part, nb, s); break;
-        case 6: rc = launch_group<64, 4>(w, b, pos, gamma_dev, part, nb, s); break;
-        case 7: rc = launch_group<64, 8>(w, b, pos, gamma_dev, part, nb, s); break;
-        default:
-            hipLaunchKernelGGL(k_wa_large, dim3(nb), dim3(kBlock), 0, s, w.slot_node, w.slot_off,
-                               w.net_start, pos, w.bucket_begin[b], w.bucket_begin[b + 1],
-                               gamma_dev, w.slot_grad, part);
-            rc = hipGetLastError() == hipSuccess ? ERP_OK : fail(ERP_ERR_HIP, "k_wa_large launch failed");
+        case 0: launch_group<1, 4>(d, b, nb, nj, s); break;
+        case 1: launch_group<2, 4>(d, b, nb, nj, s); break;
+        case 2: launch_group<4, 4>(d, b, nb, nj, s); break;
+        case 3: launch_group<8, 4>(d, b, nb, nj, s); break;
+        case 4: launch_group<16, 4>(d, b, nb, nj, s); break;
+        case 5: launch_group<32, 4>(d, b, nb, nj, s); break;
+        case 6: launch_group<64, 4>(d, b, nb, nj, s); break;
+        case 7: launch_group<64, 8>(d, b, nb, nj, s); break;
+        default: hipLaunchKernelGGL(k_wa_large, dim3(nb, nj), dim3(kBlock), 0, s, d, b);
         }
         if (t) t->end();
-        if (rc) return rc;
-        part += nb;
+        ERP_HIP_TRY(hipGetLastError());
     }
     return ERP_OK;
 }
 
-int launch_hpwl_dbu(const WaNets &w, const float2 *pos, int64_t num_nets, unsigned long long *out,
-                    hipStream_t s) {
+int launch_hpwl_dbu(const WaNets &w, int64_t num_nets, unsigned long long *out, hipStream_t s) {
     if (num_nets <= 0) return ERP_OK;
     hipLaunchKernelGGL(k_hpwl_dbu, dim3(blocks(num_nets, kBlock)), dim3(kBlock), 0, s, w.slot_node,
-                       w.slot_off, w.net_start, pos, num_nets, out);
+                       w.slot_off, w.net_start, w.pos, num_nets, out);
     ERP_HIP_TRY(hipGetLastError());
     return ERP_OK;
 }
