@@ -80,6 +80,81 @@ def cpu_baseline(d, grid, td, iters, threads):
     return iters / dt, dt
 
 
+def serving_bench(n_requests, rate, check_every, max_active, max_iterations):
+    """Serving form: the request stream (log-uniform 2k-150k cells, grids 64-512,
+    stop at overflow < 0.1) placed (1) one after another alone, (2) all submitted at
+    once to the continuous-batching server, (3) with Poisson arrivals at `rate`.
+    Returns the serving object of the JSON line."""
+    import numpy as np
+    from erplacer import workloads as W
+    from erplacer.placement import Placement
+    from erplacer.server import Server
+    stop = W.SERVING["stop_overflow"]
+    td = W.SERVING["target_density"]
+    t0 = time.perf_counter()
+    reqs = W.serving_requests(n_requests, rate_per_s=rate)
+    print(f"[serving] generated {len(reqs)} requests in {time.perf_counter() - t0:.1f}s "
+          f"({sum(d.num_movable for _, d, _ in reqs)} cells)", file=sys.stderr, flush=True)
+
+    # (1) sequential plain solves (context creation included)
+    seq_t, seq_res = [], []
+    for _, d, g in reqs:
+        t = time.perf_counter()
+        with Placement(d, g, target_density=td) as P:
+            P.set_store_gradients(False)
+            info = P.run_until(max_iterations, stop, check_every)
+            dbu = P.final_hpwl_dbu()
+        seq_t.append(time.perf_counter() - t)
+        seq_res.append((info["iteration"], dbu))
+    seq_total = sum(seq_t)
+    print(f"[serving] sequential: {seq_total:.2f}s", file=sys.stderr, flush=True)
+
+    def served(arrivals):
+        with Server(max_active=max_active, check_every=check_every) as srv:
+            t_start = time.perf_counter()
+            ids = []
+            for (arr, d, g) in zip(arrivals, [r[1] for r in reqs], [r[2] for r in reqs]):
+                if arr > 0:
+                    dt = arr - (time.perf_counter() - t_start)
+                    if dt > 0:
+                        time.sleep(dt)
+                ids.append(srv.submit(d, g, max_iterations, stop, target_density=td))
+            res = [srv.wait(i) for i in ids]
+            makespan = time.perf_counter() - t_start
+            st = srv.stats()
+        return res, makespan, st
+
+    res_sat, mk_sat, st_sat = served([0.0] * len(reqs))
+    print(f"[serving] saturated: {mk_sat:.2f}s stats {st_sat}", file=sys.stderr, flush=True)
+    arrivals = [a for a, _, _ in reqs]
+    res_poi, mk_poi, st_poi = served(arrivals)
+    print(f"[serving] poisson {rate}/s: {mk_poi:.2f}s stats {st_poi}", file=sys.stderr, flush=True)
+
+    same = all(r["status"] == 0 and (r["iterations"], r["hpwl_dbu"]) == s
+               for res in (res_sat, res_poi) for r, s in zip(res, seq_res))
+    lat = np.array([r["t_done"] - r["t_submit"] for r in res_poi])
+    # sequential server under the same arrivals: FIFO, one solve at a time
+    done, seq_lat = 0.0, []
+    for a, t in zip(arrivals, seq_t):
+        done = max(done, a) + t
+        seq_lat.append(done - a)
+    seq_lat = np.array(seq_lat)
+    conv = sum(1 for r in res_sat if r["converged"])
+    return {
+        "requests": len(reqs), "arrival_rate_per_s": rate, "check_every": check_every,
+        "max_active": max_active, "max_iterations": max_iterations, "stop_overflow": stop,
+        "converged": conv,
+        "placements_per_min": round(60 * len(reqs) / mk_sat, 1),
+        "sequential_placements_per_min": round(60 * len(reqs) / seq_total, 1),
+        "speedup_vs_sequential": round(seq_total / mk_sat, 3),
+        "p50_s": round(float(np.percentile(lat, 50)), 4), "p99_s": round(float(np.percentile(lat, 99)), 4),
+        "sequential_p50_s": round(float(np.percentile(seq_lat, 50)), 4),
+        "sequential_p99_s": round(float(np.percentile(seq_lat, 99)), 4),
+        "poisson_makespan_s": round(mk_poi, 3), "max_batch": st_sat["max_batch"],
+        "results_identical_to_alone": bool(same),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +170,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="only the timed loop (for rocprofv3 runs)")
+    ap.add_argument("--serving-requests", type=int, default=512,
+                    help="requests of the serving measurement (0 skips it)")
+    ap.add_argument("--serving-rate", type=float, default=20.0)
+    ap.add_argument("--serving-check-every", type=int, default=10)
+    ap.add_argument("--serving-max-active", type=int, default=64)
+    ap.add_argument("--serving-max-iterations", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -161,6 +242,12 @@ def main():
                          f"OpenMP restatement of the Rayon path ({cdt:.1f}s)"}
         print(f"[bench] cpu baseline {cps:.4f} it/s on {threads} threads", file=sys.stderr, flush=True)
 
+    serving = None
+    if args.serving_requests > 0:
+        serving = serving_bench(args.serving_requests, args.serving_rate, args.serving_check_every,
+                                args.serving_max_active, args.serving_max_iterations)
+        print(f"[serving] {serving}", file=sys.stderr, flush=True)
+
     out = {
         "metric": METRIC, "value": round(ips, 2), "unit": "GP iterations/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 4),
@@ -172,7 +259,7 @@ def main():
                    "parallelism": "single GPU"},
         "final": {"hpwl": info["hpwl"], "overflow": info["overflow"], "hpwl_dbu": hpwl_dbu,
                   "iteration": info["iteration"]},
-        "roofline": roof, "kernels": kernels, "cpu_baseline": cpu,
+        "roofline": roof, "kernels": kernels, "cpu_baseline": cpu, "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

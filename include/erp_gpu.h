@@ -162,6 +162,64 @@ int erp_final_hpwl_dbu(erp_ctx *ctx, int64_t *hpwl_dbu);
 int erp_set_timing(erp_ctx *ctx, int enabled);
 int erp_get_timing(erp_ctx *ctx, double *ms_per_kernel, int32_t capacity, int32_t *count,
                    const char **names);
+/* Device memory a context holds (erp_ctx_device_bytes) and the upper estimate
+ * used for admission before one exists (erp_estimate_device_bytes). */
+int erp_ctx_device_bytes(erp_ctx *ctx, int64_t *bytes);
+int erp_estimate_device_bytes(const erp_design *design, const erp_config *config, int64_t *bytes);
+
+/* ----------------------------------------------------------------- serving
+ * No counterpart in the reference (main.rs places one design per process).
+ * A persistent loop that runs many independent placements on one GPU with
+ * continuous batching: submitted jobs are admitted in arrival order while
+ * job slots and the device-memory budget allow, keep their state resident,
+ * and jobs with the same bin grid advance together in one captured hipGraph
+ * of `check_every` batched iterations.  After each graph every job's overflow
+ * is read; a job stops like erp_run_until(max_iterations, stop_overflow,
+ * check_every) and its result is bit-identical to that call on the design
+ * alone (FFT spectral form). */
+typedef struct erp_server erp_server;
+
+typedef struct {
+    int64_t hbm_budget_bytes;   /* sum of admitted jobs' estimated device memory */
+    int32_t max_active;         /* jobs resident at once */
+    int32_t check_every;        /* iterations between convergence checks */
+    int32_t device;
+} erp_server_config;
+
+typedef struct {
+    int64_t job_id;
+    int32_t status;             /* ERP_OK or the error that ended the job */
+    int32_t converged;          /* overflow < stop_overflow */
+    int64_t iterations;
+    double hpwl, overflow;      /* scalars of the last step */
+    int64_t hpwl_dbu;           /* exact final HPWL (erp_final_hpwl_dbu) */
+    double t_submit, t_admit, t_done;  /* seconds on the server's clock */
+} erp_job_result;
+
+typedef struct {
+    int64_t pending, active, finished, bytes_in_use;
+    int64_t rounds, batched_launches, max_batch;
+    double clock;               /* seconds since erp_server_create */
+} erp_server_stat;
+
+void erp_server_config_default(erp_server_config *config);
+int erp_server_create(const erp_server_config *config, erp_server **out);
+/* Stops the worker after the running round; queued jobs are dropped. */
+int erp_server_destroy(erp_server *server);
+/* Copies the design; returns at once with the job id. */
+int erp_server_submit(erp_server *server, const erp_design *design, const erp_config *config,
+                      int64_t max_iterations, double stop_overflow, int64_t *job_id);
+/* Results of jobs finished since the last poll (up to capacity). */
+int erp_server_poll(erp_server *server, erp_job_result *out, int32_t capacity, int32_t *count);
+/* Blocks until the job has finished. */
+int erp_server_wait(erp_server *server, int64_t job_id, erp_job_result *out);
+/* Blocks until no job is queued or running. */
+int erp_server_drain(erp_server *server);
+/* Final positions of a finished job (num_nodes each). */
+int erp_server_get_positions(erp_server *server, int64_t job_id, float *x, float *y);
+/* Forget a finished job's result and positions. */
+int erp_server_release(erp_server *server, int64_t job_id);
+int erp_server_stats(erp_server *server, erp_server_stat *stats);
 
 /* --------------------------------------------------- leaf kernels (host buffers)
  * Stand-alone forms of the reference's leaf functions, for per-kernel parity.

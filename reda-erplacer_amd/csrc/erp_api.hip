@@ -865,6 +865,20 @@ int erp_get_timing(erp_ctx *ctx, double *ms, int32_t capacity, int32_t *count, c
     return timer_kernel_names(ctx, ms, capacity, count, names);
 }
 
+int erp_ctx_device_bytes(erp_ctx *ctx, int64_t *bytes) {
+    if (!ctx || !bytes) return fail(ERP_ERR_INVALID, "null argument");
+    *bytes = (int64_t)ctx_device_bytes(ctx);
+    return ERP_OK;
+}
+
+int erp_estimate_device_bytes(const erp_design *d, const erp_config *cfg, int64_t *bytes) {
+    if (!d || !cfg || !bytes) return fail(ERP_ERR_INVALID, "null argument");
+    int rc = validate(d, cfg);
+    if (rc) return rc;
+    *bytes = (int64_t)estimate_ctx_bytes(d, cfg);
+    return ERP_OK;
+}
+
 // ------------------------------------------------------------------ leaf kernels
 namespace {
 struct LeafScope {

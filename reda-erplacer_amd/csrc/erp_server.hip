@@ -1,0 +1,498 @@
+// Serving form: a persistent request loop that runs many placements at once on
+// one GPU with continuous batching.
+//
+// Requests (designs) are submitted from any thread and queue up.  A worker
+// thread owns the GPU: it admits queued jobs in arrival order while the job
+// slots and the HBM budget allow (a job that does not fit waits, later smaller
+// ones may pass it), creates their contexts (state stays resident in device
+// memory), runs each new job's first `check_every` iterations on its own, and
+// then adds it to the batch of its bin grid.  Each batch is one hipGraph that
+// holds `check_every` iterations of every member, every kernel launched once for
+// all members (job = blockIdx.y).  After each round the worker reads every
+// job's scalars; a job whose overflow fell below its stop threshold, that
+// reached its iteration limit, or whose scalars became non-finite is retired:
+// its positions and exact HPWL are copied out, its device memory is freed and
+// queued jobs are admitted into the freed room, changing the batches.
+//
+// A job's kernels only read and write its own buffers and reduce in a fixed
+// order, so its result is bit-identical alone (erp_run_until with the same
+// check interval) and in any mix.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+
+#include "erp_ctx.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// Host copy of a submitted design (the caller's arrays may go away).
+struct HostDesign {
+    std::vector<float> x, y, w, h, npins, pox, poy;
+    std::vector<int32_t> pin_node, net_start, net_pins, node_pin_start, node_pins;
+    erp_design d{};
+    void copy_from(const erp_design *s) {
+        auto cp = [](auto &v, const auto *p, int64_t n) { v.assign(p, p + n); };
+        cp(x, s->x, s->num_nodes);
+        cp(y, s->y, s->num_nodes);
+        cp(w, s->w, s->num_nodes);
+        cp(h, s->h, s->num_nodes);
+        cp(npins, s->num_pins, s->num_nodes);
+        cp(pox, s->pin_offset_x, s->num_pins_total);
+        cp(poy, s->pin_offset_y, s->num_pins_total);
+        cp(pin_node, s->pin_node, s->num_pins_total);
+        cp(net_start, s->net_start, s->num_nets + 1);
+        cp(net_pins, s->net_pins, s->num_nets > 0 ? s->net_start[s->num_nets] : 0);
+        cp(node_pin_start, s->node_pin_start, s->num_nodes + 1);
+        cp(node_pins, s->node_pins, s->node_pin_start[s->num_nodes]);
+        d = *s;
+        d.x = x.data(); d.y = y.data(); d.w = w.data(); d.h = h.data(); d.num_pins = npins.data();
+        d.pin_offset_x = pox.data(); d.pin_offset_y = poy.data(); d.pin_node = pin_node.data();
+        d.net_start = net_start.data(); d.net_pins = net_pins.data();
+        d.node_pin_start = node_pin_start.data(); d.node_pins = node_pins.data();
+    }
+};
+
+struct Job {
+    int64_t id = 0;
+    HostDesign design;
+    erp_config cfg{};
+    int64_t max_iterations = 0;
+    double stop_overflow = 0.0;
+    size_t bytes = 0;
+    erp_ctx *ctx = nullptr;
+    int64_t iterations = 0;
+    erp_job_result result{};
+    std::vector<float> px, py;      // final positions
+    bool done = false;
+};
+
+struct Group {
+    int m = 0, n = 0;
+    std::vector<Job *> members;
+    bool dirty = true;
+    DevBuf table;                   // JobDesc[members]
+    BatchDims dims;
+    hipGraphExec_t graph = nullptr;
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+    }
+};
+
+}  // namespace
+
+struct erp_server {
+    erp_server_config cfg{};
+    hipStream_t stream = nullptr;
+    Clock::time_point t0;
+    std::mutex mu;
+    std::condition_variable cv, cv_done;
+    std::deque<std::unique_ptr<Job>> pending;
+    std::map<int64_t, std::unique_ptr<Job>> active;     // admitted, not finished
+    std::map<int64_t, std::unique_ptr<Job>> finished;   // results not yet released
+    std::vector<int64_t> unpolled;
+    std::map<std::pair<int, int>, Group> groups;
+    int64_t next_id = 1;
+    int64_t admitting = 0;        // job between the queue and the active set
+    size_t bytes_in_use = 0;
+    int64_t rounds = 0, batched_launches = 0, max_batch = 0;
+    bool stop = false;
+    int worker_rc = ERP_OK;
+    std::string worker_error;
+    std::thread worker;
+    double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+};
+
+namespace {
+
+void fail_job(erp_server *s, Job *j, int rc) {
+    j->result.status = rc;
+    j->result.t_done = s->now();
+    j->done = true;
+}
+
+// Retire a finished job: copy results out and free its device memory.
+void retire(erp_server *s, Job *j, const erp_step_info &info) {
+    erp_ctx *c = j->ctx;
+    j->result.iterations = j->iterations;
+    j->result.hpwl = info.hpwl;
+    j->result.overflow = info.overflow;
+    j->result.converged = info.overflow < j->stop_overflow;
+    j->px.resize((size_t)c->N);
+    j->py.resize((size_t)c->N);
+    int rc = erp_get_positions(c, j->px.data(), j->py.data());
+    int64_t dbu = 0;
+    if (!rc) rc = erp_final_hpwl_dbu(c, &dbu);
+    j->result.hpwl_dbu = dbu;
+    j->result.status = rc;
+    j->result.t_done = s->now();
+    erp_ctx_destroy(c);
+    j->ctx = nullptr;
+    j->done = true;
+}
+
+bool finished_now(const Job *j, const erp_step_info &info) {
+    return info.nonfinite || info.overflow < j->stop_overflow || j->iterations >= j->max_iterations;
+}
+
+// Admit queued jobs that fit; each runs its first chunk alone.
+void admit(erp_server *s) {
+    for (;;) {
+        std::unique_ptr<Job> job;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if ((int)s->active.size() >= s->cfg.max_active) return;
+            auto it = std::find_if(s->pending.begin(), s->pending.end(), [&](const std::unique_ptr<Job> &j) {
+                return s->bytes_in_use + j->bytes <= (size_t)s->cfg.hbm_budget_bytes;
+            });
+            if (it == s->pending.end()) return;
+            job = std::move(*it);
+            s->pending.erase(it);
+            s->bytes_in_use += job->bytes;
+            s->admitting = job->id;
+        }
+        Job *j = job.get();
+        j->result.t_admit = s->now();
+        int rc = erp_ctx_create(&j->design.d, &j->cfg, &j->ctx);
+        erp_step_info info{};
+        if (!rc) {
+            erp_set_store_gradients(j->ctx, 0);
+            const int64_t first = std::min<int64_t>(s->cfg.check_every, j->max_iterations);
+            rc = erp_run(j->ctx, first, &info);
+            j->iterations = first;
+            const int64_t left = j->max_iterations - j->iterations;
+            if (!rc && !finished_now(j, info) && left < s->cfg.check_every) {
+                rc = erp_run(j->ctx, left, &info);
+                j->iterations += left;
+            }
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->admitting = 0;
+        if (rc) {
+            if (j->ctx) erp_ctx_destroy(j->ctx);
+            j->ctx = nullptr;
+            s->bytes_in_use -= j->bytes;
+            fail_job(s, j, rc);
+            s->unpolled.push_back(j->id);
+            s->finished[j->id] = std::move(job);
+            s->cv_done.notify_all();
+            continue;
+        }
+        if (finished_now(j, info)) {
+            retire(s, j, info);
+            s->bytes_in_use -= j->bytes;
+            s->unpolled.push_back(j->id);
+            s->finished[j->id] = std::move(job);
+            s->cv_done.notify_all();
+            continue;
+        }
+        Group &g = s->groups[{j->ctx->m, j->ctx->n}];
+        g.m = j->ctx->m;
+        g.n = j->ctx->n;
+        g.members.push_back(j);
+        g.dirty = true;
+        s->active[j->id] = std::move(job);
+    }
+}
+
+int rebuild(erp_server *s, Group &g) {
+    g.drop_graph();
+    g.dirty = false;
+    if (g.members.empty()) return ERP_OK;
+    std::vector<JobDesc> tab;
+    g.dims = BatchDims();
+    for (Job *j : g.members) {
+        tab.push_back(j->ctx->job);
+        g.dims.add(j->ctx->job);
+    }
+    int rc = g.table.alloc(sizeof(JobDesc) * tab.size());
+    if (rc) return rc;
+    ERP_HIP_TRY(hipMemcpyAsync(g.table.p, tab.data(), sizeof(JobDesc) * tab.size(), hipMemcpyHostToDevice, s->stream));
+    ERP_HIP_TRY(hipStreamSynchronize(s->stream));
+    const int nj = (int)tab.size();
+    ERP_HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < s->cfg.check_every && !rc; ++k)
+        rc = launch_batched_iteration(g.table.as<JobDesc>(), g.dims, nj, s->stream);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(s->stream, &graph);
+    if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+    if (e != hipSuccess) return fail(ERP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&g.graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(ERP_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->max_batch = std::max<int64_t>(s->max_batch, nj);
+    return ERP_OK;
+}
+
+// One round: every batch advances check_every iterations, then every active
+// job is checked.
+int round(erp_server *s) {
+    std::vector<Job *> running;
+    for (auto &kv : s->groups) {
+        Group &g = kv.second;
+        if (g.dirty) {
+            int rc = rebuild(s, g);
+            if (rc) return rc;
+        }
+        if (!g.graph) continue;
+        ERP_HIP_TRY(hipGraphLaunch(g.graph, s->stream));
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            ++s->batched_launches;
+        }
+        for (Job *j : g.members) running.push_back(j);
+    }
+    if (running.empty()) return ERP_OK;
+    std::vector<DevState> st(running.size());
+    for (size_t k = 0; k < running.size(); ++k)
+        ERP_HIP_TRY(hipMemcpyAsync(&st[k], running[k]->ctx->state.p, sizeof(DevState), hipMemcpyDeviceToHost, s->stream));
+    ERP_HIP_TRY(hipStreamSynchronize(s->stream));
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        ++s->rounds;
+    }
+    for (size_t k = 0; k < running.size(); ++k) {
+        Job *j = running[k];
+        j->iterations += s->cfg.check_every;
+        j->ctx->iter_host += s->cfg.check_every;
+        erp_step_info info{};
+        info.hpwl = st[k].hpwl;
+        info.overflow = st[k].overflow;
+        info.nonfinite = st[k].nonfinite;
+        const int64_t left = j->max_iterations - j->iterations;
+        if (!finished_now(j, info) && left < s->cfg.check_every) {
+            // the last, shorter chunk runs alone (as erp_run_until would)
+            int rc = erp_run(j->ctx, left, &info);
+            if (rc) return rc;
+            j->iterations += left;
+        }
+        if (!finished_now(j, info)) continue;
+        retire(s, j, info);
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (auto &kv : s->groups) {
+            auto &mem = kv.second.members;
+            auto it = std::find(mem.begin(), mem.end(), j);
+            if (it != mem.end()) {
+                mem.erase(it);
+                kv.second.dirty = true;
+            }
+        }
+        s->bytes_in_use -= j->bytes;
+        s->unpolled.push_back(j->id);
+        s->finished[j->id] = std::move(s->active[j->id]);
+        s->active.erase(j->id);
+        s->cv_done.notify_all();
+    }
+    return ERP_OK;
+}
+
+void worker_main(erp_server *s) {
+    if (hipSetDevice(s->cfg.device) != hipSuccess) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->worker_rc = ERP_ERR_HIP;
+        s->worker_error = "hipSetDevice failed in the serving worker";
+        s->cv_done.notify_all();
+        return;
+    }
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv.wait(lk, [&] { return s->stop || !s->pending.empty() || !s->active.empty(); });
+            if (s->stop) break;
+        }
+        admit(s);
+        int rc = round(s);
+        if (rc) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->worker_rc = rc;
+            s->worker_error = erp_last_error();
+            // fail everything still queued or running; the server stops serving
+            for (auto &kv : s->active) {
+                if (kv.second->ctx) erp_ctx_destroy(kv.second->ctx);
+                kv.second->ctx = nullptr;
+                fail_job(s, kv.second.get(), rc);
+                s->unpolled.push_back(kv.first);
+                s->finished[kv.first] = std::move(kv.second);
+            }
+            s->active.clear();
+            for (auto &j : s->pending) {
+                fail_job(s, j.get(), rc);
+                s->unpolled.push_back(j->id);
+                s->finished[j->id] = std::move(j);
+            }
+            s->pending.clear();
+            for (auto &kv : s->groups) kv.second.drop_graph();
+            s->groups.clear();
+            s->cv_done.notify_all();
+            break;
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (auto it = s->groups.begin(); it != s->groups.end();) {
+            if (it->second.members.empty()) {
+                it->second.drop_graph();
+                it = s->groups.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        if (s->stop) break;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void erp_server_config_default(erp_server_config *cfg) {
+    if (!cfg) return;
+    cfg->hbm_budget_bytes = (int64_t)32 << 30;
+    cfg->max_active = 64;
+    cfg->check_every = 10;
+    cfg->device = 0;
+}
+
+int erp_server_create(const erp_server_config *cfg, erp_server **out) {
+    if (!cfg || !out) return fail(ERP_ERR_INVALID, "null argument");
+    if (cfg->max_active <= 0 || cfg->check_every <= 0 || cfg->hbm_budget_bytes <= 0)
+        return fail(ERP_ERR_INVALID, "bad server config");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(ERP_ERR_NO_DEVICE, "no HIP device");
+    if (cfg->device < 0 || cfg->device >= ndev) return fail(ERP_ERR_INVALID, "bad device ordinal");
+    ERP_HIP_TRY(hipSetDevice(cfg->device));
+    erp_server *s = new erp_server();
+    s->cfg = *cfg;
+    s->t0 = Clock::now();
+    if (hipStreamCreate(&s->stream) != hipSuccess) {
+        delete s;
+        return fail(ERP_ERR_HIP, "hipStreamCreate failed");
+    }
+    s->worker = std::thread(worker_main, s);
+    *out = s;
+    return ERP_OK;
+}
+
+int erp_server_submit(erp_server *s, const erp_design *d, const erp_config *cfg,
+                      int64_t max_iterations, double stop_overflow, int64_t *job_id) {
+    if (!s || !d || !cfg || !job_id) return fail(ERP_ERR_INVALID, "null argument");
+    if (max_iterations <= 0) return fail(ERP_ERR_INVALID, "max_iterations must be positive");
+    if (cfg->spectral_mode != ERP_SPECTRAL_FFT)
+        return fail(ERP_ERR_UNSUPPORTED, "the serving loop batches the FFT spectral form");
+    auto job = std::make_unique<Job>();
+    job->design.copy_from(d);
+    job->cfg = *cfg;
+    job->cfg.device = s->cfg.device;
+    job->max_iterations = max_iterations;
+    job->stop_overflow = stop_overflow;
+    job->bytes = estimate_ctx_bytes(d, cfg);
+    if (job->bytes > (size_t)s->cfg.hbm_budget_bytes)
+        return fail(ERP_ERR_INVALID, "design needs more device memory than the server budget");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop || s->worker_rc) return fail(ERP_ERR_INVALID, "server is stopping or failed");
+    job->id = s->next_id++;
+    job->result.job_id = job->id;
+    job->result.t_submit = s->now();
+    *job_id = job->id;
+    s->pending.push_back(std::move(job));
+    s->cv.notify_all();
+    return ERP_OK;
+}
+
+int erp_server_poll(erp_server *s, erp_job_result *out, int32_t capacity, int32_t *count) {
+    if (!s || !count || (capacity > 0 && !out)) return fail(ERP_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    int32_t k = 0;
+    while (k < capacity && !s->unpolled.empty()) {
+        const int64_t id = s->unpolled.front();
+        s->unpolled.erase(s->unpolled.begin());
+        out[k++] = s->finished[id]->result;
+    }
+    *count = k;
+    return ERP_OK;
+}
+
+int erp_server_wait(erp_server *s, int64_t job_id, erp_job_result *out) {
+    if (!s) return fail(ERP_ERR_INVALID, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    const bool known = s->finished.count(job_id) || s->active.count(job_id) || s->admitting == job_id ||
+                       std::any_of(s->pending.begin(), s->pending.end(),
+                                   [&](const std::unique_ptr<Job> &j) { return j->id == job_id; });
+    if (!known) return fail(ERP_ERR_INVALID, "unknown or released job id");
+    s->cv_done.wait(lk, [&] {
+        auto it = s->finished.find(job_id);
+        return (it != s->finished.end() && it->second->done) || s->worker_rc != ERP_OK;
+    });
+    auto it = s->finished.find(job_id);
+    if (it == s->finished.end()) return fail(s->worker_rc, s->worker_error);
+    if (out) *out = it->second->result;
+    return ERP_OK;
+}
+
+int erp_server_drain(erp_server *s) {
+    if (!s) return fail(ERP_ERR_INVALID, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] {
+        return (s->pending.empty() && s->active.empty() && s->admitting == 0) || s->worker_rc != ERP_OK;
+    });
+    if (s->worker_rc) return fail(s->worker_rc, s->worker_error);
+    return ERP_OK;
+}
+
+int erp_server_get_positions(erp_server *s, int64_t job_id, float *x, float *y) {
+    if (!s || !x || !y) return fail(ERP_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->finished.find(job_id);
+    if (it == s->finished.end() || !it->second->done) return fail(ERP_ERR_INVALID, "job not finished");
+    const Job &j = *it->second;
+    std::copy(j.px.begin(), j.px.end(), x);
+    std::copy(j.py.begin(), j.py.end(), y);
+    return ERP_OK;
+}
+
+int erp_server_release(erp_server *s, int64_t job_id) {
+    if (!s) return fail(ERP_ERR_INVALID, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->finished.erase(job_id);
+    return ERP_OK;
+}
+
+int erp_server_stats(erp_server *s, erp_server_stat *st) {
+    if (!s || !st) return fail(ERP_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    st->pending = (int64_t)s->pending.size();
+    st->active = (int64_t)s->active.size();
+    st->finished = (int64_t)s->finished.size();
+    st->bytes_in_use = (int64_t)s->bytes_in_use;
+    st->rounds = s->rounds;
+    st->batched_launches = s->batched_launches;
+    st->max_batch = s->max_batch;
+    st->clock = s->now();
+    return ERP_OK;
+}
+
+int erp_server_destroy(erp_server *s) {
+    if (!s) return ERP_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->pending.clear();          // queued jobs are dropped
+        s->cv.notify_all();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipSetDevice(s->cfg.device);
+    for (auto &kv : s->active)
+        if (kv.second->ctx) erp_ctx_destroy(kv.second->ctx);
+    for (auto &kv : s->groups) kv.second.drop_graph();
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return ERP_OK;
+}
+
+}  // extern "C"

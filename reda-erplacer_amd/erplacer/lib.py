@@ -61,6 +61,30 @@ class StepInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active", C.c_int32),
+                ("check_every", C.c_int32), ("device", C.c_int32)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("job_id", C.c_int64), ("status", C.c_int32), ("converged", C.c_int32),
+                ("iterations", C.c_int64), ("hpwl", C.c_double), ("overflow", C.c_double),
+                ("hpwl_dbu", C.c_int64), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStat(C.Structure):
+    _fields_ = [("pending", C.c_int64), ("active", C.c_int64), ("finished", C.c_int64),
+                ("bytes_in_use", C.c_int64), ("rounds", C.c_int64),
+                ("batched_launches", C.c_int64), ("max_batch", C.c_int64), ("clock", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -84,9 +108,10 @@ def load(path: str | None = None):
     lib = C.CDLL(path)
     lib.erp_last_error.restype = C.c_char_p
     lib.erp_config_default.restype = None
+    lib.erp_server_config_default.restype = None
     for name in declared_functions():
         fn = getattr(lib, name)
-        if name not in ("erp_last_error", "erp_config_default"):
+        if name not in ("erp_last_error", "erp_config_default", "erp_server_config_default"):
             fn.restype = C.c_int
     _lib = lib
     return lib

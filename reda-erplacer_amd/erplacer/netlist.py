@@ -206,16 +206,28 @@ def generate(num_cells: int, num_nets: int, degree_cap: int, num_macros: int,
         cells = np.empty((k, d), np.int64)
         cells[:, 0] = anchors[idx]
         win = max(locality, 2 * int(d))
+        seen = [set([int(a)]) for a in cells[:, 0]] if d > 12 else None
         for j in range(1, int(d)):
             cand = (anchors[idx] + rng.integers(-win, win + 1, size=k)) % C
             # resolve duplicates against earlier columns by linear probing
-            for _ in range(64):
-                dup = np.zeros(k, bool)
-                for jj in range(j):
-                    dup |= cand == cells[:, jj]
-                if not dup.any():
-                    break
-                cand[dup] = (cand[dup] + 1) % C
+            # (at most 64 steps of +1 per pin)
+            if seen is None:
+                for _ in range(64):
+                    dup = np.zeros(k, bool)
+                    for jj in range(j):
+                        dup |= cand == cells[:, jj]
+                    if not dup.any():
+                        break
+                    cand[dup] = (cand[dup] + 1) % C
+            else:   # the same probing per net with a set (high degrees)
+                for r in range(k):
+                    c, s = int(cand[r]), seen[r]
+                    for _ in range(64):
+                        if c not in s:
+                            break
+                        c = (c + 1) % C
+                    cand[r] = c
+                    s.add(c)
             cells[:, j] = cand
         starts = net_start[idx]
         pos = starts[:, None] + np.arange(d)[None, :]

@@ -155,6 +155,12 @@ class Placement:
         L.check(self._lib.erp_final_hpwl_dbu(self._h, C.byref(v)))
         return v.value
 
+    def device_bytes(self) -> int:
+        """Device memory this context holds."""
+        v = C.c_int64()
+        L.check(self._lib.erp_ctx_device_bytes(self._h, C.byref(v)))
+        return v.value
+
     def set_timing(self, enabled: bool):
         L.check(self._lib.erp_set_timing(self._h, C.c_int(int(enabled))))
 

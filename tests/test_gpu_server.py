@@ -1,0 +1,95 @@
+"""Serving loop: continuous batching of independent placements on one GPU.
+
+Each job's result must be bit-identical to the same design placed alone with
+erp_run_until(max_iterations, stop_overflow, check_every): the batched kernels
+only touch a job's own buffers and reduce in a fixed order, and the server
+checks convergence at the same iteration counts.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CHECK = 10
+
+
+def _designs():
+    from erplacer.netlist import generate
+    specs = [(1500, 64, 11), (3000, 64, 12), (2200, 64, 13), (6000, 128, 14), (9000, 128, 15),
+             (4000, 64, 16), (12000, 128, 17)]
+    out = []
+    for cells, grid, seed in specs:
+        d = generate(cells, int(cells * 1.05), 32, 6, 0.1, 0.7, seed, name=f"job{seed}")
+        out.append((d, grid))
+    return out
+
+
+def _alone(d, grid, max_it, stop):
+    from erplacer.placement import Placement
+    with Placement(d, grid, target_density=1.0) as P:
+        info = P.run_until(max_it, stop, CHECK)
+        x, y = P.positions()
+        return info, x, y, P.final_hpwl_dbu()
+
+
+@pytest.mark.parametrize("max_active", [3, 16])
+def test_alone_equals_mix(gpu_lib, max_active):
+    from erplacer.server import Server
+    jobs = _designs()
+    max_it, stop = 237, 0.25        # not a multiple of CHECK: the last chunk is shorter
+    alone = [_alone(d, g, max_it, stop) for d, g in jobs]
+    with Server(max_active=max_active, check_every=CHECK) as srv:
+        ids = [srv.submit(d, g, max_it, stop, target_density=1.0) for d, g in jobs]
+        res = {i: srv.wait(i) for i in ids}
+        st = srv.stats()
+        for (d, g), jid, (info, x, y, dbu) in zip(jobs, ids, alone):
+            r = res[jid]
+            assert r["status"] == 0
+            assert r["iterations"] == info["iteration"], (jid, r, info)
+            assert r["hpwl"] == info["hpwl"] and r["overflow"] == info["overflow"]
+            assert r["hpwl_dbu"] == dbu
+            sx, sy = srv.positions(jid)
+            assert np.array_equal(sx, x) and np.array_equal(sy, y), jid
+            srv.release(jid)
+    assert st["max_batch"] >= 2          # same-grid jobs really ran together
+    assert st["max_batch"] <= max_active
+    print("server stats", st)
+
+
+def test_budget_admission(gpu_lib):
+    """A budget that fits two jobs keeps at most two resident."""
+    from erplacer.netlist import generate
+    from erplacer.server import Server, estimate_device_bytes
+    jobs = [(generate(3000, 3150, 32, 6, 0.1, 0.7, 40 + k), 64) for k in range(5)]
+    est = [estimate_device_bytes(d, g, target_density=1.0) for d, g in jobs]
+    need = max(est)
+    assert 3 * min(est) > 2 * need + 1      # no third job fits
+    with Server(hbm_budget_bytes=2 * need + 1, max_active=64, check_every=CHECK) as srv:
+        ids = [srv.submit(d, g, 120, 0.3, target_density=1.0) for d, g in jobs]
+        srv.drain()
+        got = srv.poll()
+        assert sorted(r["job_id"] for r in got) == sorted(ids)
+        assert all(r["status"] == 0 for r in got)
+        assert srv.stats()["max_batch"] <= 2
+        # intervals [t_admit, t_done] never overlap more than twice
+        ev = sorted([(r["t_admit"], 1) for r in got] + [(r["t_done"], -1) for r in got],
+                    key=lambda e: (e[0], e[1]))
+        live = peak = 0
+        for _, k in ev:
+            live += k
+            peak = max(peak, live)
+        assert peak <= 2
+    with Server(hbm_budget_bytes=need // 2, check_every=CHECK) as srv:
+        from erplacer import lib as L
+        with pytest.raises(L.ErpError):
+            srv.submit(jobs[0][0], jobs[0][1], 100, 0.1, target_density=1.0)
+
+
+def test_estimate_covers_allocation(gpu_lib):
+    from erplacer.placement import Placement
+    from erplacer.server import estimate_device_bytes
+    for d, g in _designs()[:4]:
+        est = estimate_device_bytes(d, g, target_density=1.0)
+        with Placement(d, g, target_density=1.0) as P:
+            real = P.device_bytes()
+        assert real <= est < 2 * real + (4 << 20), (real, est)

@@ -108,3 +108,22 @@ def test_invalid_design_rejected_before_device_use():
     bad = L.default_config(num_bins_x=100)
     assert lib.erp_ctx_create(ctypes.byref(ds), ctypes.byref(bad), ctypes.byref(h)) == L.ERP_ERR_UNSUPPORTED
     assert b"power of two" in lib.erp_last_error()
+
+
+def test_server_arguments_checked_before_device_use():
+    """The serving entry points reject bad configurations and estimate memory
+    without touching a device."""
+    from erplacer import server as S
+    lib = L.load()
+    cfg = L.ServerConfig()
+    lib.erp_server_config_default(ctypes.byref(cfg))
+    assert cfg.max_active > 0 and cfg.check_every > 0 and cfg.hbm_budget_bytes > 0
+    h = ctypes.c_void_p()
+    bad = L.ServerConfig(hbm_budget_bytes=1 << 30, max_active=0, check_every=10, device=0)
+    assert lib.erp_server_create(ctypes.byref(bad), ctypes.byref(h)) == L.ERP_ERR_INVALID
+    dd = generate(2000, 2100, 16, 4, 0.1, 0.7, 3)
+    small = S.estimate_device_bytes(dd, 64)
+    big = S.estimate_device_bytes(dd, 512)
+    assert 0 < small < big
+    # one int64 + eight float maps per bin at least
+    assert big - small >= (512 * 512 - 64 * 64) * (8 + 8 * 4)
