@@ -45,11 +45,15 @@ def kernel_bytes(name, d, grid, cells_pins):
     table = {
         # per pin: slot node (4) + pin offset (8) + instance position gather (8) + grad write (8)
         "wa_2pin": n2 * 2 * 28,
-        # permuted cell id (4) + position gather (8) + size/offset record (16) + ratio (4);
-        # int64 map written once
-        "scatter": C * 32 + B * 8,
-        "sort_keys": C * 20,
-        "sort_place": C * 48,
+        # sorted record (16) + ratio (4) + offset (8) + new order (4) read; order (4),
+        # constants (16) and ratio (4) written back for the next sort; int64 map
+        # written once
+        "scatter": C * 56 + B * 8,
+        # order (4), position gather (8), constants (16); key (4) + corner (8) written
+        "sort_keys": C * 40,
+        # key, order, corner, constants, ratio read (36); order, record, ratio,
+        # offset written in sorted order (32)
+        "sort_place": C * 68,
         # read q_mov, q_fix (16), clear q_mov (8), write density (4) and row transform (4)
         "pass1_row_dct": B * 32,
         # read row transform (4), write three column-inverse maps (12)

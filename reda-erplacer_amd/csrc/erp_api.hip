@@ -242,7 +242,7 @@ size_t erp::estimate_ctx_bytes(const erp_design *d, const erp_config *cfg) {
     const size_t N = (size_t)d->num_nodes, C = (size_t)d->num_movable, P = (size_t)d->num_pins_total;
     const size_t E = (size_t)d->num_nets, B = (size_t)cfg->num_bins_x * (size_t)cfg->num_bins_y;
     const size_t per_node = 5 * 4 + 3 * 8 + 4;                       // DB copies, pos/csize/coff, ratio
-    const size_t per_cell = 6 * 8 + 2 * 4 + 8 + 4 * 4 + 16 + 4 + 4;  // state, area/pins, hi, sort
+    const size_t per_cell = 6 * 8 + 2 * 4 + 8 + 4 * 4 + 16 + 4 + 4 + 36;  // state, area/pins, hi, sort
     const size_t per_pin = 4 + 8 + 8 + 4;                             // slots, grads, cell->slot
     const size_t per_bin = 2 * 8 + 9 * 4;                             // q_mov/q_fix + float maps
     const size_t partials = 16 * (P / 64 + C / 128 + (size_t)cfg->num_bins_x) + 64 * 1024;
@@ -259,7 +259,7 @@ size_t erp::ctx_device_bytes(const erp_ctx *c) {
                             &c->pot, &c->auv, &c->hpwl_part, &c->ovf_part, &c->energy_part,
                             &c->part_a, &c->part_b, &c->state, &c->gamma_override, &c->hpwl_dbu,
                             &c->keys, &c->perm, &c->perm_out, &c->hist, &c->cursor, &c->rec,
-                            &c->rratio, &c->djob, &c->djob_gamma, &c->tm.fft_tw, &c->tm.dct_tw,
+                            &c->rratio, &c->sstat, &c->sratio, &c->sll, &c->roff, &c->djob, &c->djob_gamma, &c->tm.fft_tw, &c->tm.dct_tw,
                             &c->tn.fft_tw, &c->tn.dct_tw};
     size_t total = 0;
     for (const DevBuf *b : bufs) total += b->bytes;
@@ -321,11 +321,27 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
         if (e < d->n_2nets && b - a != 2)
             return fail(ERP_ERR_INVALID, "net " + std::to_string(e) + " below n_2nets is not two-pin");
     }
+    // Net order on the device: two-pin nets first, then multi-pin nets by
+    // degree; inside each group by the lowest instance id they touch, so that
+    // consecutive nets (one wavefront's work) gather nearby instances and the
+    // per-instance gradient gather reads nearby pin slots.  The order only
+    // changes the rounding of the FP64 HPWL sum.
+    std::vector<int32_t> low_node((size_t)E, INT32_MAX);
+    for (int64_t e = 0; e < E; ++e)
+        for (int64_t k = d->net_start[e]; k < d->net_start[e + 1]; ++k) {
+            const int32_t pin = d->net_pins[k];
+            if (pin < 0 || pin >= P) return fail(ERP_ERR_INVALID, "pin id out of range");
+            low_node[e] = std::min(low_node[e], d->pin_node[pin]);
+        }
     for (int64_t e = 0; e < d->n_2nets; ++e) order.push_back(e);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return low_node[a] < low_node[b]; });
     std::vector<int64_t> multi;
     for (int64_t e = d->n_2nets; e < E; ++e) multi.push_back(e);
     std::stable_sort(multi.begin(), multi.end(), [&](int64_t a, int64_t b) {
-        return (d->net_start[a + 1] - d->net_start[a]) < (d->net_start[b + 1] - d->net_start[b]);
+        const int64_t da = d->net_start[a + 1] - d->net_start[a];
+        const int64_t db = d->net_start[b + 1] - d->net_start[b];
+        return da != db ? da < db : low_node[a] < low_node[b];
     });
     int64_t bucket_count[kWaBuckets] = {0};
     for (int64_t e : multi) {
@@ -522,6 +538,10 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     TRY(ctx->perm_out.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(C, 1)));
     TRY(ctx->rec.alloc(sizeof(float4) * (size_t)std::max<int64_t>(C, 1)));
     TRY(ctx->rratio.alloc(sizeof(float) * (size_t)std::max<int64_t>(C, 1)));
+    TRY(ctx->sstat.alloc(sizeof(float4) * (size_t)std::max<int64_t>(C, 1)));
+    TRY(ctx->sratio.alloc(sizeof(float) * (size_t)std::max<int64_t>(C, 1)));
+    TRY(ctx->sll.alloc(sizeof(float2) * (size_t)std::max<int64_t>(C, 1)));
+    TRY(ctx->roff.alloc(sizeof(float2) * (size_t)std::max<int64_t>(C, 1)));
     TRY(launch_iota(ctx->perm.as<int32_t>(), C, s));
     TRY(launch_iota(ctx->perm_out.as<int32_t>(), C, s));
     TRY(ctx->hist.alloc(sizeof(int32_t) * (size_t)kSortTiles));
@@ -544,6 +564,10 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
         sb.perm_out = ctx->perm_out.as<int32_t>();
         sb.rec = ctx->rec.as<float4>();
         sb.rratio = ctx->rratio.as<float>();
+        sb.sstat = ctx->sstat.as<float4>();
+        sb.sratio = ctx->sratio.as<float>();
+        sb.ll = ctx->sll.as<float2>();
+        sb.roff = ctx->roff.as<float2>();
     }
     for (DevBuf *b : {&ctx->hpwl_part, &ctx->ovf_part, &ctx->energy_part, &ctx->part_a, &ctx->part_b, &ctx->hist})
         if (hipMemsetAsync(b->p, 0, b->bytes, s) != hipSuccess) { erp_ctx_destroy(ctx); return fail(ERP_ERR_HIP, "hipMemsetAsync failed"); }
@@ -551,6 +575,8 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
                            N, C, ctx->bw, ctx->bh, ctx->t, die_w, die_h, ctx->csize.as<float2>(),
                            ctx->coff.as<float2>(), ctx->ratio.as<float>(), ctx->area.as<float>(),
                            ctx->npins.as<float>(), ctx->hi.as<float2>(), s));
+    TRY(launch_init_sort_static(ctx->csize.as<float2>(), ctx->coff.as<float2>(), ctx->ratio.as<float>(),
+                                C, ctx->sstat.as<float4>(), ctx->sratio.as<float>(), s));
     TRY(reset_ctx(ctx));
     TRY(launch_zero_i64(ctx->q_fix.as<int64_t>(), ctx->B, s));
     TRY(launch_fixed_map(ctx->pos.as<float2>() + C, ctx->csize.as<float2>() + C, N - C, ctx->m,

@@ -27,7 +27,7 @@ struct erp_ctx {
     DevBuf q_mov, q_fix, rho, rowt, p1, p2, p3, fx, fy, pot, auv;
     DevBuf hpwl_part, ovf_part, energy_part, part_a, part_b;
     DevBuf state, gamma_override, hpwl_dbu;
-    DevBuf keys, perm, perm_out, hist, cursor, rec, rratio;
+    DevBuf keys, perm, perm_out, hist, cursor, rec, rratio, sstat, sratio, sll, roff;
     SpecTables tm, tn;
     MfmaPlan *mfma = nullptr;
     JobDesc job{};                 // host copy of the job descriptor

@@ -69,6 +69,9 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const JobDesc *__restrict__ 
     const float *__restrict__ rratio = sb.rratio;
     const int32_t *__restrict__ perm_src = copy_perm ? sb.perm_out : nullptr;
     int32_t *__restrict__ perm_dst = copy_perm ? sb.perm : nullptr;
+    const float2 *__restrict__ roff = sb.roff;
+    float4 *__restrict__ sstat = sb.sstat;
+    float *__restrict__ sratio = sb.sratio;
     int64_t *__restrict__ q = jd[blockIdx.y].pb.q_mov;
     const int m = sb.m, n = sb.n;
     const float bw = sb.bw, bh = sb.bh;
@@ -83,13 +86,18 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const JobDesc *__restrict__ 
         bxl[j] = bxh[j] = byl[j] = byh[j] = 0;
         xi[j] = yi[j] = sx[j] = sy[j] = r[j] = 0.f;
         if (i < num_cells) {
-            if (perm_dst) perm_dst[i] = perm_src[i];
             const float4 v = rec[i];
             xi[j] = v.x;
             yi[j] = v.y;
             sx[j] = v.z;
             sy[j] = v.w;
             r[j] = rratio[i];
+            if (perm_dst) {          // the sorted order becomes the next sort's input order
+                perm_dst[i] = perm_src[i];
+                const float2 o = roff[i];
+                sstat[i] = make_float4(v.z, v.w, o.x, o.y);
+                sratio[i] = r[j];
+            }
             bin_range(xi[j], sx[j], inv_bw, m, bxl[j], bxh[j]);
             bin_range(yi[j], sy[j], inv_bh, n, byl[j], byh[j]);
             if (bxh[j] > bxl[j] && byh[j] > byl[j]) {
@@ -184,8 +192,9 @@ __global__ void __launch_bounds__(kBlock) k_sort_keys(const JobDesc *__restrict_
     if ((int64_t)blockIdx.x * kScatterCells >= num_cells) return;
     const int32_t *__restrict__ perm = sb.perm;
     const float2 *__restrict__ pos = sb.pos;
-    const float2 *__restrict__ coff = sb.coff;
+    const float4 *__restrict__ sstat = sb.sstat;
     int32_t *__restrict__ keys = sb.keys;
+    float2 *__restrict__ ll = sb.ll;
     int32_t *__restrict__ hist = sb.hist;
     const int m = sb.m, n = sb.n;
     const float inv_bw = 1.f / sb.bw, inv_bh = 1.f / sb.bh;
@@ -199,9 +208,12 @@ __global__ void __launch_bounds__(kBlock) k_sort_keys(const JobDesc *__restrict_
         int k = -1;
         if (valid) {
             const int32_t c = perm[i];
-            const float2 p = pos[c], o = coff[c];
-            k = sort_key(p.x + o.x, p.y + o.y, inv_bw, inv_bh, m, n, shift);
+            const float2 p = pos[c];
+            const float4 st = sstat[i];
+            const float2 l = make_float2(p.x + st.z, p.y + st.w);
+            k = sort_key(l.x, l.y, inv_bw, inv_bh, m, n, shift);
             keys[i] = k;
+            ll[i] = l;
         }
         // a wavefront whose cells share one tile adds its count once
         const int k0 = __shfl(k, 0);
@@ -251,14 +263,14 @@ __global__ void __launch_bounds__(kBlock) k_sort_place(const JobDesc *__restrict
     if ((int64_t)blockIdx.x * kScatterCells >= num_cells) return;
     const int32_t *__restrict__ perm = sb.perm;
     const int32_t *__restrict__ keys = sb.keys;
-    const float2 *__restrict__ pos = sb.pos;
-    const float2 *__restrict__ csize = sb.csize;
-    const float2 *__restrict__ coff = sb.coff;
-    const float *__restrict__ ratio = sb.ratio;
+    const float2 *__restrict__ ll = sb.ll;
+    const float4 *__restrict__ sstat = sb.sstat;
+    const float *__restrict__ sratio = sb.sratio;
     int32_t *__restrict__ cursor = sb.cursor;
     int32_t *__restrict__ perm_out = sb.perm_out;
     float4 *__restrict__ rec = sb.rec;
     float *__restrict__ rratio = sb.rratio;
+    float2 *__restrict__ roff = sb.roff;
     for (int t = threadIdx.x; t < ntiles; t += kBlock) lh[t] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kScatterCells;
@@ -290,11 +302,12 @@ __global__ void __launch_bounds__(kBlock) k_sort_place(const JobDesc *__restrict
         if (key[j] >= 0) {
             const int64_t i = base + threadIdx.x + (int64_t)j * kBlock;
             const int dst = lh[key[j]] + rank[j];
-            const int32_t c = perm[i];
-            const float2 p = pos[c], sz = csize[c], o = coff[c];
-            perm_out[dst] = c;
-            rec[dst] = make_float4(p.x + o.x, p.y + o.y, sz.x, sz.y);
-            rratio[dst] = ratio[c];
+            const float2 l = ll[i];
+            const float4 st = sstat[i];
+            perm_out[dst] = perm[i];
+            rec[dst] = make_float4(l.x, l.y, st.x, st.y);
+            rratio[dst] = sratio[i];
+            roff[dst] = make_float2(st.z, st.w);
         }
     }
 }
@@ -307,6 +320,18 @@ __global__ void k_pack_records(const float2 *__restrict__ pos, const float2 *__r
     if (i < num_cells) {
         const float2 p = pos[i], sz = csize[i], o = coff[i];
         rec[i] = make_float4(p.x + o.x, p.y + o.y, sz.x, sz.y);
+    }
+}
+
+// Sort constants in identity order (the order of a fresh perm).
+__global__ void k_init_sort_static(const float2 *__restrict__ csize, const float2 *__restrict__ coff,
+                                   const float *__restrict__ ratio, int64_t num_cells,
+                                   float4 *__restrict__ sstat, float *__restrict__ sratio) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < num_cells) {
+        const float2 sz = csize[i], o = coff[i];
+        sstat[i] = make_float4(sz.x, sz.y, o.x, o.y);
+        sratio[i] = ratio[i];
     }
 }
 
@@ -400,6 +425,15 @@ int launch_sort(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s, Ke
     if (t) t->begin("sort_place");
     hipLaunchKernelGGL(k_sort_place, dim3(nb, nj), dim3(kBlock), 0, s, d, g.ntiles);
     if (t) t->end();
+    ERP_HIP_TRY(hipGetLastError());
+    return ERP_OK;
+}
+
+int launch_init_sort_static(const float2 *csize, const float2 *coff, const float *ratio,
+                            int64_t num_cells, float4 *sstat, float *sratio, hipStream_t s) {
+    if (num_cells <= 0) return ERP_OK;
+    hipLaunchKernelGGL(k_init_sort_static, dim3((unsigned)((num_cells + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, csize, coff, ratio, num_cells, sstat, sratio);
     ERP_HIP_TRY(hipGetLastError());
     return ERP_OK;
 }

@@ -26,7 +26,15 @@ struct SortBufs {
     int32_t *keys, *hist, *cursor;   // keys [C], hist/cursor [kSortTiles]
     float4 *rec;         // [C] sorted scatter records
     float *rratio;       // [C]
+    // per-cell constants carried along the order so that the sort reads them
+    // sequentially: {clamped w, h, offset x, y} and ratio in the order of perm
+    float4 *sstat;
+    float *sratio;
+    float2 *ll;          // [C] clamped lower-left corners in the order of perm (sort temp)
+    float2 *roff;        // [C] clamp offsets in sorted order
 };
+int launch_init_sort_static(const float2 *csize, const float2 *coff, const float *ratio,
+                            int64_t num_cells, float4 *sstat, float *sratio, hipStream_t s);
 int launch_pack_records(const float2 *pos, const float2 *csize, const float2 *coff,
                         int64_t num_cells, float4 *rec, hipStream_t s);
 int launch_iota(int32_t *v, int64_t count, hipStream_t s);
