@@ -28,9 +28,6 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "GP iters/s @1M cells & placements/min + p50/p99 latency @ HPWL within 0.5%"
 
 
-WA_BUCKETS = [("wa_deg4", 2, 4), ("wa_deg8", 4, 8), ("wa_deg16", 8, 16), ("wa_deg32", 16, 32),
-              ("wa_deg64", 32, 64), ("wa_deg128", 64, 128), ("wa_deg256", 128, 256),
-              ("wa_deg512", 256, 512), ("wa_large", 512, 1 << 30)]
 
 
 def kernel_bytes(name, d, grid, cells_pins):
@@ -42,9 +39,14 @@ def kernel_bytes(name, d, grid, cells_pins):
     B = grid * grid
     n2 = int(d.n_2nets)
     multi = deg[n2:]
+    P2 = n2 * 2
+    narrow = multi[multi <= 256]
+    wide = multi[multi > 256]
     table = {
-        # per pin: slot node (4) + pin offset (8) + instance position gather (8) + grad write (8)
-        "wa_2pin": n2 * 2 * 28,
+        # per pin: slot node (4) + pin offset (8) + instance position gather (8) + grad
+        # write (8); per multi-pin net its CSR start (8)
+        "wa_narrow": (P2 + int(narrow.sum())) * 28 + int(narrow.size) * 8,
+        "wa_wide": int(wide.sum()) * 28 + int(wide.size) * 8,
         # sorted record (16) + ratio (4) + offset (8) + new order (4) read; order (4),
         # constants (16) and ratio (4) written back for the next sort; int64 map
         # written once
@@ -67,9 +69,6 @@ def kernel_bytes(name, d, grid, cells_pins):
         # position, previous position, gradient, clamp bound read; three writes
         "update": C * 56,
     }
-    for nm, lo, hi in WA_BUCKETS:
-        sel = multi[(multi > lo) & (multi <= hi)] if nm != "wa_deg4" else multi[multi <= hi]
-        table[nm] = int(sel.sum()) * 28 + int(sel.size) * 8
     return table.get(name, 0)
 
 

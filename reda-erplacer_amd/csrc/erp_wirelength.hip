@@ -48,11 +48,11 @@ __device__ __forceinline__ float2 wa2(float a, float b, float ig) {
     return make_float2(ga, gb);
 }
 
-__global__ void __launch_bounds__(kBlock) k_wa_2pin(const JobDesc *__restrict__ jd) {
-    const WaNets &w = jd[blockIdx.y].wa;
+// Workgroup `blk` of the two-pin nets.
+__device__ __forceinline__ void wa_2pin(const WaNets &w, int64_t blk) {
     const int64_t n2 = w.n2;
-    if ((int64_t)blockIdx.x * kBlock >= n2) return;    // this job needs fewer workgroups
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (blk * kBlock >= n2) return;                    // this job needs fewer workgroups
+    const int64_t e = blk * blockDim.x + threadIdx.x;
     double h = 0.0;
     if (e < n2) {
         const float ig = 1.f / (float)(*w.gamma);
@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(kBlock) k_wa_2pin(const JobDesc *__restrict__ 
         reinterpret_cast<float4 *>(w.slot_grad)[e] = make_float4(gx.x, gy.x, gx.y, gy.y);
     }
     const double sum = block_sum_f64(h);
-    if (threadIdx.x == 0) w.hpwl_part[w.part_offset[0] + blockIdx.x] = sum;
+    if (threadIdx.x == 0) w.hpwl_part[w.part_offset[0] + blk] = sum;
 }
 
 // One net per group of G lanes (G divides 64), each lane holding up to K of the
@@ -74,17 +74,16 @@ __global__ void __launch_bounds__(kBlock) k_wa_2pin(const JobDesc *__restrict__ 
 // three passes of hpwl.rs:248-334 run on registers, and the max/min and the
 // exponential sums are reduced with shuffles inside the group.
 template <int G, int K>
-__global__ void __launch_bounds__(kBlock) k_wa_group(const JobDesc *__restrict__ jd, int bucket) {
-    const WaNets &w = jd[blockIdx.y].wa;
+__device__ __forceinline__ void wa_group(const WaNets &w, int bucket, int64_t blk) {
     const int64_t e0 = w.bucket_begin[bucket], e1 = w.bucket_begin[bucket + 1];
-    if ((int64_t)blockIdx.x * kBlock >= (e1 - e0) * G) return;   // uniform per workgroup
+    if (blk * kBlock >= (e1 - e0) * G) return;   // uniform per workgroup
     const int32_t *__restrict__ slot_node = w.slot_node;
     const float2 *__restrict__ slot_off = w.slot_off;
     const int32_t *__restrict__ net_start = w.net_start;
     const float2 *__restrict__ pos = w.pos;
     float2 *__restrict__ slot_grad = w.slot_grad;
     const double *gamma = w.gamma;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = blk * blockDim.x + threadIdx.x;
     const int64_t e = e0 + t / G;
     const int r = (int)(t & (G - 1));
     const bool valid = e < e1;
@@ -156,7 +155,7 @@ __global__ void __launch_bounds__(kBlock) k_wa_group(const JobDesc *__restrict__
         }
     }
     const double sum = block_sum_f64(h);
-    if (threadIdx.x == 0) w.hpwl_part[w.part_offset[bucket + 1] + blockIdx.x] = sum;
+    if (threadIdx.x == 0) w.hpwl_part[w.part_offset[bucket + 1] + blk] = sum;
 }
 
 __device__ __forceinline__ float wave_max(float v) {
@@ -176,10 +175,9 @@ __device__ __forceinline__ float wave_fsum(float v) {
 }
 
 // One net per wavefront (4 per workgroup).
-__global__ void __launch_bounds__(kBlock) k_wa_large(const JobDesc *__restrict__ jd, int bucket) {
-    const WaNets &w = jd[blockIdx.y].wa;
+__device__ __forceinline__ void wa_large(const WaNets &w, int bucket, int64_t blk) {
     const int64_t e0 = w.bucket_begin[bucket], e1 = w.bucket_begin[bucket + 1];
-    if ((int64_t)blockIdx.x * (kBlock / 64) >= e1 - e0) return;  // uniform per workgroup
+    if (blk * (kBlock / 64) >= e1 - e0) return;  // uniform per workgroup
     const int32_t *__restrict__ slot_node = w.slot_node;
     const float2 *__restrict__ slot_off = w.slot_off;
     const int32_t *__restrict__ net_start = w.net_start;
@@ -187,7 +185,7 @@ __global__ void __launch_bounds__(kBlock) k_wa_large(const JobDesc *__restrict__
     float2 *__restrict__ slot_grad = w.slot_grad;
     const double *gamma = w.gamma;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t e = e0 + (int64_t)blockIdx.x * (kBlock / 64) + wave;
+    const int64_t e = e0 + blk * (kBlock / 64) + wave;
     double h = 0.0;
     if (e < e1) {
         const float ig = 1.f / (float)(*gamma);
@@ -227,7 +225,42 @@ __global__ void __launch_bounds__(kBlock) k_wa_large(const JobDesc *__restrict__
         }
     }
     const double sum = block_sum_f64(h);
-    if (threadIdx.x == 0) w.hpwl_part[w.part_offset[bucket + 1] + blockIdx.x] = sum;
+    if (threadIdx.x == 0) w.hpwl_part[w.part_offset[bucket + 1] + blk] = sum;
+}
+
+// One launch for several net groups: the grid is cut into segments of
+// workgroups, each segment running one group (-1: two-pin nets, b >= 0: degree
+// bucket b), so small buckets overlap instead of each draining the GPU alone.
+struct WaPlan {
+    int nseg;
+    int bucket[kWaBuckets + 1];
+    unsigned begin[kWaBuckets + 2];
+};
+
+__global__ void __launch_bounds__(kBlock) k_wa_narrow(const JobDesc *__restrict__ jd, WaPlan plan) {
+    const WaNets &w = jd[blockIdx.y].wa;
+    int sg = 0;
+    while (sg + 1 < plan.nseg && blockIdx.x >= plan.begin[sg + 1]) ++sg;
+    const int64_t blk = (int64_t)blockIdx.x - plan.begin[sg];
+    switch (plan.bucket[sg]) {
+    case -1: wa_2pin(w, blk); break;
+    case 0: wa_group<1, 4>(w, 0, blk); break;
+    case 1: wa_group<2, 4>(w, 1, blk); break;
+    case 2: wa_group<4, 4>(w, 2, blk); break;
+    case 3: wa_group<8, 4>(w, 3, blk); break;
+    case 4: wa_group<16, 4>(w, 4, blk); break;
+    case 5: wa_group<32, 4>(w, 5, blk); break;
+    default: wa_group<64, 4>(w, 6, blk); break;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_wa_wide(const JobDesc *__restrict__ jd, WaPlan plan) {
+    const WaNets &w = jd[blockIdx.y].wa;
+    int sg = 0;
+    while (sg + 1 < plan.nseg && blockIdx.x >= plan.begin[sg + 1]) ++sg;
+    const int64_t blk = (int64_t)blockIdx.x - plan.begin[sg];
+    if (plan.bucket[sg] == 7) wa_group<64, 8>(w, 7, blk);
+    else wa_large(w, 8, blk);
 }
 
 // Exact HPWL: pin coordinates rounded to integer DBU (nearest-even).
@@ -259,21 +292,14 @@ inline unsigned blocks(int64_t count, int per) { return (unsigned)((count + per 
 }  // namespace
 
 // Degree buckets of the multi-pin nets: bucket b holds degrees in
-// (kWaDegree[b-1], kWaDegree[b]] and runs k_wa_group<G, K> with G*K >= degree.
+// (kWaDegree[b-1], kWaDegree[b]] and runs wa_group<G, K> with G*K >= degree.
 const int kWaDegree[kWaBuckets] = {4, 8, 16, 32, 64, 128, 256, 512, 1 << 30};
 
 namespace {
 // lanes per net of bucket b (the last bucket loops with one wavefront per net)
 constexpr int kWaLanes[kWaBuckets] = {1, 2, 4, 8, 16, 32, 64, 64, 64};
-const char *const kWaNames[kWaBuckets] = {"wa_deg4", "wa_deg8", "wa_deg16", "wa_deg32",
-                                          "wa_deg64", "wa_deg128", "wa_deg256", "wa_deg512",
-                                          "wa_large"};
 unsigned bucket_blocks(int64_t nets, int b) {
     return b == kWaBuckets - 1 ? blocks(nets, kBlock / 64) : blocks(nets * kWaLanes[b], kBlock);
-}
-template <int G, int K>
-void launch_group(const JobDesc *d, int b, unsigned nb, int nj, hipStream_t s) {
-    hipLaunchKernelGGL((k_wa_group<G, K>), dim3(nb, nj), dim3(kBlock), 0, s, d, b);
 }
 }  // namespace
 
@@ -297,28 +323,35 @@ int wa_partials(WaNets &w) {
 }
 
 int launch_wa(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s, KernelTimer *t) {
-    const unsigned b2 = blocks(bd.max_n2, kBlock);
-    if (b2) {
-        if (t) t->begin("wa_2pin");
-        hipLaunchKernelGGL(k_wa_2pin, dim3(b2, nj), dim3(kBlock), 0, s, d);
+    // heavier workgroups first: high-degree buckets, then the small ones
+    WaPlan narrow{}, wide{};
+    unsigned total = 0;
+    for (int b = 6; b >= -1; --b) {
+        const unsigned nb = b < 0 ? blocks(bd.max_n2, kBlock) : bucket_blocks(bd.max_bucket[b], b);
+        if (!nb) continue;
+        narrow.bucket[narrow.nseg] = b;
+        narrow.begin[narrow.nseg++] = total;
+        total += nb;
+    }
+    narrow.begin[narrow.nseg] = total;
+    if (total) {
+        if (t) t->begin("wa_narrow");
+        hipLaunchKernelGGL(k_wa_narrow, dim3(total, nj), dim3(kBlock), 0, s, d, narrow);
         if (t) t->end();
         ERP_HIP_TRY(hipGetLastError());
     }
-    for (int b = 0; b < kWaBuckets; ++b) {
+    total = 0;
+    for (int b = kWaBuckets - 1; b >= 7; --b) {
         const unsigned nb = bucket_blocks(bd.max_bucket[b], b);
         if (!nb) continue;
-        if (t) t->begin(kWaNames[b]);
-        switch (b) {
-        case 0: launch_group<1, 4>(d, b, nb, nj, s); break;
-        case 1: launch_group<2, 4>(d, b, nb, nj, s); break;
-        case 2: launch_group<4, 4>(d, b, nb, nj, s); break;
-        case 3: launch_group<8, 4>(d, b, nb, nj, s); break;
-        case 4: launch_group<16, 4>(d, b, nb, nj, s); break;
-        case 5: launch_group<32, 4>(d, b, nb, nj, s); break;
-        case 6: launch_group<64, 4>(d, b, nb, nj, s); break;
-        case 7: launch_group<64, 8>(d, b, nb, nj, s); break;
-        default: hipLaunchKernelGGL(k_wa_large, dim3(nb, nj), dim3(kBlock), 0, s, d, b);
-        }
+        wide.bucket[wide.nseg] = b;
+        wide.begin[wide.nseg++] = total;
+        total += nb;
+    }
+    wide.begin[wide.nseg] = total;
+    if (total) {
+        if (t) t->begin("wa_wide");
+        hipLaunchKernelGGL(k_wa_wide, dim3(total, nj), dim3(kBlock), 0, s, d, wide);
         if (t) t->end();
         ERP_HIP_TRY(hipGetLastError());
     }
