@@ -246,7 +246,12 @@ size_t erp::estimate_ctx_bytes(const erp_design *d, const erp_config *cfg) {
     const size_t per_pin = 4 + 8 + 8 + 4;                             // slots, grads, cell->slot
     const size_t per_bin = 2 * 8 + 9 * 4;                             // q_mov/q_fix + float maps
     const size_t partials = 16 * (P / 64 + C / 128 + (size_t)cfg->num_bins_x) + 64 * 1024;
-    return N * per_node + C * per_cell + P * per_pin + E * 4 + B * per_bin + partials +
+    size_t mfma = 0;
+    if (cfg->spectral_mode == ERP_SPECTRAL_MFMA_BF16X3) {   // operator planes + 4 scratch maps
+        const size_t hm = (size_t)cfg->num_bins_x / 2, hn = (size_t)cfg->num_bins_y / 2;
+        mfma = 24 * (hm * hm + (hm == hn ? 0 : hn * hn)) + 16 * B + 8 * B / 4096 * 4;
+    }
+    return N * per_node + C * per_cell + P * per_pin + E * 4 + B * per_bin + partials + mfma +
            2 * 8 * (size_t)(cfg->num_bins_x + cfg->num_bins_y) + (1u << 20);
 }
 
@@ -261,7 +266,7 @@ size_t erp::ctx_device_bytes(const erp_ctx *c) {
                             &c->keys, &c->perm, &c->perm_out, &c->hist, &c->cursor, &c->rec,
                             &c->rratio, &c->sstat, &c->sratio, &c->sll, &c->roff, &c->djob, &c->djob_gamma, &c->tm.fft_tw, &c->tm.dct_tw,
                             &c->tn.fft_tw, &c->tn.dct_tw};
-    size_t total = 0;
+    size_t total = mfma_plan_bytes(c->mfma);
     for (const DevBuf *b : bufs) total += b->bytes;
     return total;
 }
@@ -299,6 +304,8 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     if (cfg->device < 0 || cfg->device >= ndev) return fail(ERP_ERR_INVALID, "bad device ordinal");
     if (cfg->spectral_mode != ERP_SPECTRAL_FFT && cfg->spectral_mode != ERP_SPECTRAL_MFMA_BF16X3)
         return fail(ERP_ERR_UNSUPPORTED, "unknown spectral mode");
+    if (cfg->spectral_mode == ERP_SPECTRAL_MFMA_BF16X3 && !mfma_supported(cfg->num_bins_x, cfg->num_bins_y))
+        return fail(ERP_ERR_UNSUPPORTED, "the MFMA spectral form needs at least 128 bins per axis");
 
     // ----------------------------------------------------- host-side index setup
     const int64_t N = d->num_nodes, C = d->num_movable, P = d->num_pins_total, E = d->num_nets;
@@ -459,8 +466,10 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     w.hpwl_part = ctx->hpwl_part.as<double>();
 
     ctx->n_pot_part = potential_partials(ctx->m);
+    ctx->n_energy_part = ctx->spectral_mode == ERP_SPECTRAL_MFMA_BF16X3 ? mfma_energy_partials(ctx->m, ctx->n)
+                                                                      : ctx->n_pot_part;
     TRY(ctx->ovf_part.alloc(sizeof(int64_t) * (size_t)ctx->n_pot_part));
-    TRY(ctx->energy_part.alloc(sizeof(double) * (size_t)ctx->n_pot_part));
+    TRY(ctx->energy_part.alloc(sizeof(double) * (size_t)ctx->n_energy_part));
     PotentialBufs &pb = ctx->job.pb;
     pb.q_fix = ctx->q_fix.as<int64_t>();
     pb.q_mov = ctx->q_mov.as<int64_t>();
@@ -526,7 +535,7 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     sa.ovf_part = ctx->ovf_part.as<int64_t>();
     sa.n_ovf = ctx->n_pot_part;
     sa.energy_part = ctx->energy_part.as<double>();
-    sa.n_energy = ctx->n_pot_part;
+    sa.n_energy = ctx->n_energy_part;
     sa.part_a = ctx->part_a.as<double>();
     sa.part_b = ctx->part_b.as<double>();
     sa.n_cell = ctx->n_cell_part;
@@ -922,10 +931,30 @@ int leaf_spectral(const float *x, float *out, int32_t m, int32_t n, int32_t mode
     if (!x || !out) return fail(ERP_ERR_INVALID, "null argument");
     if (!check_grid(m) || !check_grid(n))
         return fail(ERP_ERR_UNSUPPORTED, "sizes must be powers of two in [64, 2048]");
-    if (mode != ERP_SPECTRAL_FFT) return fail(ERP_ERR_UNSUPPORTED, "leaf transforms run the FFT form");
+    if (mode != ERP_SPECTRAL_FFT && mode != ERP_SPECTRAL_MFMA_BF16X3)
+        return fail(ERP_ERR_UNSUPPORTED, "unknown spectral mode");
+    if (mode == ERP_SPECTRAL_MFMA_BF16X3 && !mfma_supported(m, n))
+        return fail(ERP_ERR_UNSUPPORTED, "the MFMA spectral form needs at least 128 bins per axis");
     LeafScope L;
     if (L.rc) return L.rc;
     const size_t B = (size_t)m * n;
+    if (mode == ERP_SPECTRAL_MFMA_BF16X3) {
+        DevBuf dx, dout;
+        MfmaPlan *plan = nullptr;
+        int rc;
+        if ((rc = upload(dx, x, B, L.s)) || (rc = dout.alloc(sizeof(float) * B)) ||
+            (rc = mfma_plan_create(&plan, m, n, L.s)))
+            return rc;
+        // which: 0 dct2, 1 idct2, 2 idct_idxst, 3 idxst_idct
+        rc = which == 0 ? launch_dct2_mfma(dx.as<float>(), dout.as<float>(), plan, L.s)
+                        : launch_inverse_mfma(dx.as<float>(), dout.as<float>(), which - 1, plan, L.s);
+        if (!rc && hipMemcpyAsync(out, dout.p, sizeof(float) * B, hipMemcpyDeviceToHost, L.s) != hipSuccess)
+            rc = fail(ERP_ERR_HIP, "hipMemcpyAsync failed");
+        if (!rc && hipStreamSynchronize(L.s) != hipSuccess) rc = fail(ERP_ERR_HIP, "hipStreamSynchronize failed");
+        (void)hipStreamSynchronize(L.s);
+        mfma_plan_destroy(plan);
+        return rc;
+    }
     DevBuf dx, dout, dscr;
     SpecTables tm, tn;
     int rc;

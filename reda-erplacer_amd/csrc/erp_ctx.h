@@ -33,7 +33,7 @@ struct erp_ctx {
     JobDesc job{};                 // host copy of the job descriptor
     DevBuf djob, djob_gamma;       // device copies (the second reads an explicit gamma)
     BatchDims dims;                // launch geometry of this placement alone
-    int n_hpwl_part = 0, n_pot_part = 0, n_cell_part = 0;
+    int n_hpwl_part = 0, n_pot_part = 0, n_energy_part = 0, n_cell_part = 0;
     hipGraphExec_t graph_many = nullptr, graph_one = nullptr;
     KernelTimer *timer = nullptr;
     bool timing = false;

@@ -187,3 +187,69 @@ def test_small_workload_end_state(gpu_lib):
     assert not a["nonfinite"]
     assert rel < 0.03
     assert abs(a["overflow"] - b["overflow"]) < 0.01
+
+
+def test_mfma_step0_maps(gpu_lib):
+    """The MFMA spectral form (split-BF16 transform products) in the placement
+    loop: step-0 maps against the FP64 oracle, overflow identical to the FFT form
+    (same fixed-point density), and the first steps tracking the oracle."""
+    from erplacer import lib as L
+    from erplacer.placement import Placement
+    d, _ = _design(grid=128)
+    g = 128
+    P = Placement(d, g, target_density=1.0, spectral_mode=L.SPECTRAL_MFMA_BF16X3)
+    F = Placement(d, g, target_density=1.0)
+    ref = O.Placer(d, g, target_density=1.0)
+    energy, overflow = P.compute_potential()
+    fe, fo = F.compute_potential()
+    r = ref.step()
+    assert overflow == fo
+    assert abs(overflow - r["overflow"]) < 2e-5
+    assert abs(energy / r["energy"] - 1) < TOL_MAP
+    for which_gpu, which_ref in ((L.MAP_DENSITY, O.MAP_DENSITY), (L.MAP_FIELD_X, O.MAP_FIELD_X),
+                                 (L.MAP_FIELD_Y, O.MAP_FIELD_Y), (L.MAP_POTENTIAL, O.MAP_POTENTIAL),
+                                 (L.MAP_AUV, O.MAP_AUV)):
+        err = rel_l2(P.map(which_gpu), ref.map(which_ref))
+        print(f"mfma step-0 map {which_gpu}: rel L2 {err:.2e}")
+        assert err < TOL_MAP, which_gpu
+    Q = Placement(d, g, target_density=1.0, spectral_mode=L.SPECTRAL_MFMA_BF16X3)
+    ref = O.Placer(d, g, target_density=1.0)
+    for k in range(10):
+        a = Q.step()
+        b = ref.step()
+        assert abs(a["hpwl"] / b["hpwl"] - 1) < 1e-3, k
+        assert abs(a["overflow"] - b["overflow"]) < 1e-3, k
+
+
+def test_mfma_graph_loop_matches_phase_loop(gpu_lib, small):
+    from erplacer import lib as L
+    from erplacer.placement import Placement
+    d, _ = small
+    A = Placement(d, 128, target_density=1.0, spectral_mode=L.SPECTRAL_MFMA_BF16X3)
+    B = Placement(d, 128, target_density=1.0, spectral_mode=L.SPECTRAL_MFMA_BF16X3)
+    for _ in range(20):
+        A.step()
+    B.run(20)
+    xa, ya = A.positions()
+    xb, yb = B.positions()
+    assert np.array_equal(xa, xb) and np.array_equal(ya, yb)
+
+
+def test_mfma_small_workload_end_state(gpu_lib):
+    """The whole 'small' workload with the MFMA spectral form against the FP64
+    oracle run (bounds as for the FFT form: the trajectory is chaotic)."""
+    from erplacer import lib as L
+    from erplacer import workloads as W
+    from erplacer.placement import Placement
+    d, g, iters, td = W.make("small")
+    P = Placement(d, g, target_density=td, spectral_mode=L.SPECTRAL_MFMA_BF16X3)
+    a = P.run(iters)
+    ref = O.Placer(d, g, target_density=td)
+    for _ in range(iters):
+        b = ref.step()
+    rel = abs(a["hpwl"] / b["hpwl"] - 1)
+    print(f"mfma final hpwl gpu {a['hpwl']:.6e} oracle {b['hpwl']:.6e} rel {rel:.4%}; "
+          f"overflow gpu {a['overflow']:.5f} oracle {b['overflow']:.5f}")
+    assert not a["nonfinite"]
+    assert rel < 0.03
+    assert abs(a["overflow"] - b["overflow"]) < 0.01
