@@ -158,6 +158,23 @@ def serving_bench(n_requests, rate, check_every, max_active, max_iterations):
     }
 
 
+def copy_calibration(nbytes, reps):
+    """Device-to-device hipMemcpy (the runtime's blit kernel) of a known size,
+    the reference dispatch for scaling FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0     # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(f"[bench] copy calibration: {reps} x {nbytes} bytes read + written", file=sys.stderr)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,7 +189,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true",
-                    help="only the timed loop (for rocprofv3 runs)")
+                    help="only the per-kernel timing loop, the launches whose average the "
+                         "roofline quotes (for rocprofv3 runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB device copy (torch) after the loop, the "
+                         "reference kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE scaling")
     ap.add_argument("--serving-requests", type=int, default=512,
                     help="requests of the serving measurement (0 skips it)")
     ap.add_argument("--serving-rate", type=float, default=20.0)
@@ -201,14 +222,23 @@ def main():
         P.run(args.warmup)
     P.reset()
     P.info()  # synchronise
+    if args.profile:
+        P.reset()
+        P.set_timing(True)
+        P.run(args.timing_steps or args.steps)
+        P.set_timing(False)
+        t = P.timing()
+        print(f"[bench] profile loop per-kernel avg ms: { {k: round(v, 5) for k, v in t.items()} }",
+              file=sys.stderr, flush=True)
+        if args.copy_calibration:
+            copy_calibration(256 << 20, 3)
+        return
     t0 = time.perf_counter()
     info = P.run(args.steps)   # returns after a stream synchronize
     elapsed = time.perf_counter() - t0
     ips = args.steps / elapsed
     print(f"[bench] timed {args.steps} iterations in {elapsed:.3f}s -> {ips:.1f} it/s; "
           f"hpwl {info['hpwl']:.6e} overflow {info['overflow']:.5f}", file=sys.stderr, flush=True)
-    if args.profile:
-        return
     hpwl_dbu = P.final_hpwl_dbu()
 
     # per-kernel device time with HIP events over a second timed region

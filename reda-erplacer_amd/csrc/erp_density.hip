@@ -230,27 +230,35 @@ __global__ void __launch_bounds__(kBlock) k_sort_keys(const JobDesc *__restrict_
 }
 
 __global__ void __launch_bounds__(1024) k_sort_scan(const JobDesc *__restrict__ jd, int ntiles) {
-    __shared__ int part[1024];
+    __shared__ int wsum[16];
     int32_t *__restrict__ hist = jd[blockIdx.x].sb.hist;
     int32_t *__restrict__ cursor = jd[blockIdx.x].sb.cursor;
-    const int per = (ntiles + 1023) / 1024;
+    const int per = (ntiles + 1023) / 1024;          // ntiles <= kSortTiles = 4 * 1024
     const int t0 = threadIdx.x * per;
+    int v[4] = {0, 0, 0, 0};
     int sum = 0;
-    for (int k = 0; k < per; ++k)
-        if (t0 + k < ntiles) sum += hist[t0 + k];
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {      // inclusive Hillis-Steele scan
-        const int v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (k < per && t0 + k < ntiles) { v[k] = hist[t0 + k]; sum += v[k]; }
+    // inclusive scan of the per-thread sums: within the wavefront by shuffles,
+    // then across the 16 wavefronts
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int inc = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(inc, off);
+        if (lane >= off) inc += y;
     }
-    int run = part[threadIdx.x] - sum;              // exclusive prefix of this segment
-    for (int k = 0; k < per; ++k) {
-        if (t0 + k < ntiles) {
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    int before = 0;
+    for (int w = 0; w < wave; ++w) before += wsum[w];
+    int run = before + inc - sum;                    // exclusive prefix of this thread
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k < per && t0 + k < ntiles) {
             cursor[t0 + k] = run;
-            run += hist[t0 + k];
+            run += v[k];
             hist[t0 + k] = 0;
         }
     }

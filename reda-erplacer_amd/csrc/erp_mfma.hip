@@ -42,7 +42,7 @@ enum { T_FE = 0, T_FO = 1, T_CE = 2, T_CO = 3, T_SE = 4, T_SO = 5, T_COUNT = 6 }
 
 constexpr int kBM = 64;              // output rows per workgroup (each half product)
 constexpr int kBN = 32;              // output columns per workgroup (each half product)
-constexpr int kBK = 32;              // K per step (one MFMA)
+constexpr int kBK = 64;              // K per stage (two MFMA steps of 32)
 constexpr int kLd = kBK + 8;         // LDS row length in bf16 (16-byte pad)
 constexpr int kAPlane = kBM * kLd;
 constexpr int kBPlane = kBN * kLd;
@@ -75,97 +75,125 @@ __device__ __forceinline__ const __bf16 *plane(const MfmaArgs &a, int t, int hl)
     return a.tab + (size_t)(2 * t + hl) * a.H * a.H;
 }
 
-// Global -> register staging of one K step.
+// Global -> register staging of one K stage (two stages are in flight).
 struct Stage {
-    float4 d[4];          // data operand
-    bf16x8 o[4];          // operator operand
+    float4 d[8];          // data operand
+    bf16x8 o[8];          // operator operand
 };
 
 // Data as the A operand (rows x K, K contiguous): FWD_ROW folds column k with
-// n-1-k, INV_ROW takes the two de-interleaved halves.
+// n-1-k, INV_ROW takes the two de-interleaved halves.  Thread: one row, 16 K.
 template <int MODE>
 __device__ __forceinline__ void load_a_data(const MfmaArgs &a, const float *src, int r0, int k0,
                                             Stage &s) {
-    const int row = threadIdx.x >> 2, kk = (threadIdx.x & 3) * 8;
+    const int row = threadIdx.x >> 2, kk = (threadIdx.x & 3) * 16;
     const float *p = src + (size_t)(r0 + row) * a.n;
     const float4 *x0 = reinterpret_cast<const float4 *>(p + k0 + kk);
-    const float4 *x1 = MODE == FWD_ROW ? reinterpret_cast<const float4 *>(p + a.n - 8 - k0 - kk)
+    const float4 *x1 = MODE == FWD_ROW ? reinterpret_cast<const float4 *>(p + a.n - 16 - k0 - kk)
                                        : reinterpret_cast<const float4 *>(p + a.H + k0 + kk);
-    s.d[0] = x0[0]; s.d[1] = x0[1]; s.d[2] = x1[0]; s.d[3] = x1[1];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        s.d[q] = x0[q];
+        s.d[4 + q] = x1[q];
+    }
 }
 template <int MODE>
 __device__ __forceinline__ void store_a_data(const Stage &s, __bf16 *lds) {
-    const int row = threadIdx.x >> 2, kk = (threadIdx.x & 3) * 8;
+    const int row = threadIdx.x >> 2, kk = (threadIdx.x & 3) * 16;
     const float *lo = reinterpret_cast<const float *>(&s.d[0]);
-    const float *hi = reinterpret_cast<const float *>(&s.d[2]);
-    float e[8], o[8];
+    const float *hi = reinterpret_cast<const float *>(&s.d[4]);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        if (MODE == FWD_ROW) { e[j] = lo[j] + hi[7 - j]; o[j] = lo[j] - hi[7 - j]; }
-        else { e[j] = lo[j]; o[j] = hi[j]; }
+    for (int h8 = 0; h8 < 2; ++h8) {
+        float e[8], o[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int jj = 8 * h8 + j;
+            if (MODE == FWD_ROW) { e[j] = lo[jj] + hi[15 - jj]; o[j] = lo[jj] - hi[15 - jj]; }
+            else { e[j] = lo[jj]; o[j] = hi[jj]; }
+        }
+        const int at = row * kLd + kk + 8 * h8;
+        bf16x8 vh, vl;
+        split8(e, vh, vl);
+        *reinterpret_cast<bf16x8 *>(lds + 0 * kAPlane + at) = vh;
+        *reinterpret_cast<bf16x8 *>(lds + 1 * kAPlane + at) = vl;
+        split8(o, vh, vl);
+        *reinterpret_cast<bf16x8 *>(lds + 2 * kAPlane + at) = vh;
+        *reinterpret_cast<bf16x8 *>(lds + 3 * kAPlane + at) = vl;
     }
-    bf16x8 h, l;
-    split8(e, h, l);
-    *reinterpret_cast<bf16x8 *>(lds + 0 * kAPlane + row * kLd + kk) = h;
-    *reinterpret_cast<bf16x8 *>(lds + 1 * kAPlane + row * kLd + kk) = l;
-    split8(o, h, l);
-    *reinterpret_cast<bf16x8 *>(lds + 2 * kAPlane + row * kLd + kk) = h;
-    *reinterpret_cast<bf16x8 *>(lds + 3 * kAPlane + row * kLd + kk) = l;
 }
 
 // Data as the B operand (K x cols, K along rows of the map): FWD_COL folds row
 // k with 2H-1-k, INV_COL takes the two de-interleaved halves.  Stored
-// transposed (K contiguous per column).
+// transposed (K contiguous per column).  Thread: two adjacent K rows, 4 columns.
 template <int MODE>
 __device__ __forceinline__ void load_b_data(const MfmaArgs &a, const float *src, int c0, int k0,
                                             Stage &s) {
-    const int kl = threadIdx.x >> 3, cl = (threadIdx.x & 7) * 4;
-    const int r1 = MODE == FWD_COL ? 2 * a.H - 1 - k0 - kl : a.H + k0 + kl;
-    s.d[0] = *reinterpret_cast<const float4 *>(src + (size_t)(k0 + kl) * a.n + c0 + cl);
-    s.d[1] = *reinterpret_cast<const float4 *>(src + (size_t)r1 * a.n + c0 + cl);
+    const int k2 = (threadIdx.x >> 3) * 2, cl = (threadIdx.x & 7) * 4;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int r = k0 + k2 + q;
+        const int r1 = MODE == FWD_COL ? 2 * a.H - 1 - r : a.H + r;
+        s.d[q] = *reinterpret_cast<const float4 *>(src + (size_t)r * a.n + c0 + cl);
+        s.d[2 + q] = *reinterpret_cast<const float4 *>(src + (size_t)r1 * a.n + c0 + cl);
+    }
 }
 template <int MODE>
 __device__ __forceinline__ void store_b_data(const Stage &s, __bf16 *lds) {
-    const int kl = threadIdx.x >> 3, cl = (threadIdx.x & 7) * 4;
-    const float *x = reinterpret_cast<const float *>(&s.d[0]);
-    const float *y = reinterpret_cast<const float *>(&s.d[1]);
+    const int k2 = (threadIdx.x >> 3) * 2, cl = (threadIdx.x & 7) * 4;
+    const float *x0 = reinterpret_cast<const float *>(&s.d[0]);
+    const float *x1 = reinterpret_cast<const float *>(&s.d[1]);
+    const float *y0 = reinterpret_cast<const float *>(&s.d[2]);
+    const float *y1 = reinterpret_cast<const float *>(&s.d[3]);
     __bf16 *B = lds + 4 * kAPlane;
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const float e = MODE == FWD_COL ? x[q] + y[q] : x[q];
-        const float o = MODE == FWD_COL ? x[q] - y[q] : y[q];
-        const __bf16 eh = (__bf16)e, oh = (__bf16)o;
-        const int at = (cl + q) * kLd + kl;
-        B[0 * kBPlane + at] = eh;
-        B[1 * kBPlane + at] = (__bf16)(e - (float)eh);
-        B[2 * kBPlane + at] = oh;
-        B[3 * kBPlane + at] = (__bf16)(o - (float)oh);
+        const float e0 = MODE == FWD_COL ? x0[q] + y0[q] : x0[q];
+        const float e1 = MODE == FWD_COL ? x1[q] + y1[q] : x1[q];
+        const float o0 = MODE == FWD_COL ? x0[q] - y0[q] : y0[q];
+        const float o1 = MODE == FWD_COL ? x1[q] - y1[q] : y1[q];
+        bf16x2 eh, el, oh, ol;
+        eh[0] = (__bf16)e0; eh[1] = (__bf16)e1;
+        el[0] = (__bf16)(e0 - (float)eh[0]); el[1] = (__bf16)(e1 - (float)eh[1]);
+        oh[0] = (__bf16)o0; oh[1] = (__bf16)o1;
+        ol[0] = (__bf16)(o0 - (float)oh[0]); ol[1] = (__bf16)(o1 - (float)oh[1]);
+        const int at = (cl + q) * kLd + k2;
+        *reinterpret_cast<bf16x2 *>(B + 0 * kBPlane + at) = eh;
+        *reinterpret_cast<bf16x2 *>(B + 1 * kBPlane + at) = el;
+        *reinterpret_cast<bf16x2 *>(B + 2 * kBPlane + at) = oh;
+        *reinterpret_cast<bf16x2 *>(B + 3 * kBPlane + at) = ol;
     }
 }
 
-// Operator as the A operand: 64 rows x 32 K of four planes.
+// Operator as the A operand: 64 rows x 64 K of four planes (thread: one row of
+// one plane).
 __device__ __forceinline__ void load_a_op(const MfmaArgs &a, int te, int r0, int k0, Stage &s) {
     const int pl = threadIdx.x >> 6, row = threadIdx.x & 63;
     const int t = te + (pl >> 1);   // even / odd product
     const bf16x8 *p = reinterpret_cast<const bf16x8 *>(plane(a, t, pl & 1) + (size_t)(r0 + row) * a.H + k0);
-    s.o[0] = p[0]; s.o[1] = p[1]; s.o[2] = p[2]; s.o[3] = p[3];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s.o[q] = p[q];
 }
 __device__ __forceinline__ void store_a_op(const Stage &s, __bf16 *lds) {
     const int pl = threadIdx.x >> 6, row = threadIdx.x & 63;
     bf16x8 *d = reinterpret_cast<bf16x8 *>(lds + pl * kAPlane + row * kLd);
-    d[0] = s.o[0]; d[1] = s.o[1]; d[2] = s.o[2]; d[3] = s.o[3];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) d[q] = s.o[q];
 }
-// Operator as the B operand: 32 columns x 32 K of four planes.
+// Operator as the B operand: 32 columns x 64 K of four planes (thread: half a
+// column row of one plane).
 __device__ __forceinline__ void load_b_op(const MfmaArgs &a, int te, int c0, int k0, Stage &s) {
     const int pl = threadIdx.x >> 6, col = (threadIdx.x >> 1) & 31, half = threadIdx.x & 1;
     const int t = te + (pl >> 1);
-    const bf16x8 *p = reinterpret_cast<const bf16x8 *>(plane(a, t, pl & 1) + (size_t)(c0 + col) * a.H + k0 + 16 * half);
-    s.o[0] = p[0]; s.o[1] = p[1];
+    const bf16x8 *p = reinterpret_cast<const bf16x8 *>(plane(a, t, pl & 1) + (size_t)(c0 + col) * a.H + k0 + 32 * half);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s.o[q] = p[q];
 }
 __device__ __forceinline__ void store_b_op(const Stage &s, __bf16 *lds) {
     const int pl = threadIdx.x >> 6, col = (threadIdx.x >> 1) & 31, half = threadIdx.x & 1;
-    bf16x8 *d = reinterpret_cast<bf16x8 *>(lds + 4 * kAPlane + pl * kBPlane + col * kLd + 16 * half);
-    d[0] = s.o[0]; d[1] = s.o[1];
+    bf16x8 *d = reinterpret_cast<bf16x8 *>(lds + 4 * kAPlane + pl * kBPlane + col * kLd + 32 * half);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) d[q] = s.o[q];
 }
 
 // XCD-aware tile order: consecutive workgroups go round-robin over the 8 XCDs;
@@ -176,7 +204,7 @@ __device__ __forceinline__ int tile_of_block(int b, int nb) {
 
 template <int MODE>
 __global__ void __launch_bounds__(256) k_mfma(MfmaArgs a) {
-    __shared__ __attribute__((aligned(16))) __bf16 lds[2 * kStage];
+    __shared__ __attribute__((aligned(16))) __bf16 lds[kStage];
     const int z = blockIdx.y;
     const bool sine = (a.sine_mask >> z) & 1;
     const int te = (MODE == FWD_ROW || MODE == FWD_COL) ? T_FE : (sine ? T_SE : T_CE);
@@ -188,8 +216,7 @@ __global__ void __launch_bounds__(256) k_mfma(MfmaArgs a) {
     const float *src = a.src[z];
     const int nk = a.H / kBK;
 
-    Stage st;
-    auto load = [&](int ks) {
+    auto load = [&](int ks, Stage &st) {
         const int k0 = ks * kBK;
         if (MODE == FWD_ROW || MODE == INV_ROW) {
             load_a_data<MODE>(a, src, r0, k0, st);
@@ -199,13 +226,13 @@ __global__ void __launch_bounds__(256) k_mfma(MfmaArgs a) {
             load_b_data<MODE>(a, src, c0, k0, st);
         }
     };
-    auto store = [&](__bf16 *buf) {
+    auto store = [&](const Stage &st) {
         if (MODE == FWD_ROW || MODE == INV_ROW) {
-            store_a_data<MODE>(st, buf);
-            store_b_op(st, buf);
+            store_a_data<MODE>(st, lds);
+            store_b_op(st, lds);
         } else {
-            store_a_op(st, buf);
-            store_b_data<MODE>(st, buf);
+            store_a_op(st, lds);
+            store_b_data<MODE>(st, lds);
         }
     };
 
@@ -218,29 +245,44 @@ __global__ void __launch_bounds__(256) k_mfma(MfmaArgs a) {
 #pragma unroll
         for (int rt = 0; rt < 2; ++rt) acc[p][rt] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    load(0);
-    store(lds);
-    __syncthreads();
-    for (int ks = 0; ks < nk; ++ks) {
-        __bf16 *cur = lds + (ks & 1) * kStage;
-        if (ks + 1 < nk) load(ks + 1);
+    auto compute = [&]() {
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            const __bf16 *B = cur + 4 * kAPlane + (2 * p) * kBPlane + (wc * 16 + lr) * kLd + lk;
-            const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(B);
-            const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(B + kBPlane);
+        for (int kq = 0; kq < kBK; kq += 32) {
 #pragma unroll
-            for (int rt = 0; rt < 2; ++rt) {
-                const __bf16 *A = cur + (2 * p) * kAPlane + (wr * 32 + rt * 16 + lr) * kLd + lk;
-                const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(A);
-                const bf16x8 al = *reinterpret_cast<const bf16x8 *>(A + kAPlane);
-                acc[p][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[p][rt], 0, 0, 0);
-                acc[p][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[p][rt], 0, 0, 0);
-                acc[p][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[p][rt], 0, 0, 0);
+            for (int p = 0; p < 2; ++p) {
+                const __bf16 *B = lds + 4 * kAPlane + (2 * p) * kBPlane + (wc * 16 + lr) * kLd + lk + kq;
+                const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(B);
+                const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(B + kBPlane);
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt) {
+                    const __bf16 *A = lds + (2 * p) * kAPlane + (wr * 32 + rt * 16 + lr) * kLd + lk + kq;
+                    const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(A);
+                    const bf16x8 al = *reinterpret_cast<const bf16x8 *>(A + kAPlane);
+                    acc[p][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[p][rt], 0, 0, 0);
+                    acc[p][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[p][rt], 0, 0, 0);
+                    acc[p][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[p][rt], 0, 0, 0);
+                }
             }
         }
-        if (ks + 1 < nk) store(lds + ((ks + 1) & 1) * kStage);
+    };
+
+    // two register stages in flight; one LDS stage
+    Stage s0, s1;
+    load(0, s0);
+    if (nk > 1) load(1, s1);
+    for (int ks = 0; ks < nk; ks += 2) {
+        __syncthreads();                 // the previous stage has been consumed
+        store(s0);
         __syncthreads();
+        if (ks + 2 < nk) load(ks + 2, s0);
+        compute();
+        if (ks + 1 < nk) {
+            __syncthreads();
+            store(s1);
+            __syncthreads();
+            if (ks + 3 < nk) load(ks + 3, s1);
+            compute();
+        }
     }
 
     // ------------------------------------------------------------- epilogue
@@ -315,25 +357,33 @@ __global__ void __launch_bounds__(256) k_mfma(MfmaArgs a) {
 
 // Density from the fixed-point accumulators (epotential.rs:271-285): rho,
 // overflow partials per 4 rows, movable accumulators cleared.
-__global__ void __launch_bounds__(256) k_finalize(PotentialBufs b) {
+__global__ void __launch_bounds__(1024) k_finalize(PotentialBufs b) {
     const int n = b.n;
     const size_t base = (size_t)blockIdx.x * 4 * n;
+    const longlong2 *__restrict__ qf = reinterpret_cast<const longlong2 *>(b.q_fix + base);
+    longlong2 *__restrict__ qm = reinterpret_cast<longlong2 *>(b.q_mov + base);
+    float2 *__restrict__ rho = reinterpret_cast<float2 *>(b.rho + base);
+    const int pairs = 2 * n;                  // 4 rows as int64 pairs
     int64_t ovf = 0;
-    for (int idx = threadIdx.x; idx < 4 * n; idx += 256) {
-        const size_t o = base + idx;
-        const int64_t t = b.q_mov[o] + b.q_fix[o];
-        if (!b.keep_q) b.q_mov[o] = 0;
-        ovf += t > b.coef_q ? t - b.coef_q : 0;
-        b.rho[o] = (float)((double)t * b.inv_scale) * b.inv_bin_area;
+#pragma unroll 2
+    for (int k = threadIdx.x; k < pairs; k += 1024) {
+        const longlong2 a = qm[k], f = qf[k];
+        if (!b.keep_q) qm[k] = make_longlong2(0, 0);
+        const int64_t t0 = a.x + f.x, t1 = a.y + f.y;
+        ovf += (t0 > b.coef_q ? t0 - b.coef_q : 0) + (t1 > b.coef_q ? t1 - b.coef_q : 0);
+        rho[k] = make_float2((float)((double)t0 * b.inv_scale) * b.inv_bin_area,
+                             (float)((double)t1 * b.inv_scale) * b.inv_bin_area);
     }
-    __shared__ int64_t red[256];
-    red[threadIdx.x] = ovf;
+    __shared__ int64_t red[16];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) ovf += __shfl_xor(ovf, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ovf;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t v = 0;
+        for (int w = 0; w < 16; ++w) v += red[w];
+        b.ovf_part[blockIdx.x] = v;
     }
-    if (threadIdx.x == 0) b.ovf_part[blockIdx.x] = red[0];
 }
 
 // Natural-order spectrum -> de-interleaved layout with the eps factors of the
@@ -499,7 +549,7 @@ int launch_potential_mfma(const PotentialBufs &b, MfmaPlan *p, hipStream_t s, Ke
     if (!p || p->m != b.m || p->n != b.n) return fail(ERP_ERR_INVALID, "MFMA plan does not match the grid");
     const int m = b.m, n = b.n;
     if (t) t->begin("mfma_finalize");
-    hipLaunchKernelGGL(k_finalize, dim3((unsigned)(m / 4)), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(k_finalize, dim3((unsigned)(m / 4)), dim3(1024), 0, s, b);
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
     MfmaArgs c = base_args(p, m, n);

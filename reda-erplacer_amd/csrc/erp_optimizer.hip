@@ -200,58 +200,86 @@ __global__ void __launch_bounds__(kBlock) k_update(const JobDesc *__restrict__ j
     c.prev_g[i] = g;
 }
 
-// Deterministic sums of the partial arrays by one workgroup: each thread adds a
-// fixed strided subset with four independent accumulators (so its loads are in
-// flight together), then a fixed tree combines the threads.
+// Deterministic sums of the partial arrays by one workgroup, then the FP64
+// Nesterov and scheduler updates (one workgroup per job).
 constexpr int kScalarThreads = 1024;
 
-template <class T>
-__device__ __forceinline__ T strided_sum(const T *part, int count) {
-    T v0 = 0, v1 = 0, v2 = 0, v3 = 0;
-    int k = threadIdx.x;
-    for (; k + 3 * kScalarThreads < count; k += 4 * kScalarThreads) {
-        v0 += part[k];
-        v1 += part[k + kScalarThreads];
-        v2 += part[k + 2 * kScalarThreads];
-        v3 += part[k + 3 * kScalarThreads];
-    }
-    for (; k < count; k += kScalarThreads) v0 += part[k];
-    return (v0 + v1) + (v2 + v3);
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
 }
 
 __global__ void __launch_bounds__(kScalarThreads) k_scalar(const JobDesc *__restrict__ jd, int ops) {
     const ScalarArgs &a = jd[blockIdx.x].sa;
-    __shared__ double redd[4][kScalarThreads];
-    __shared__ int64_t redi[kScalarThreads];
+    constexpr int kWaves = kScalarThreads / 64;
+    __shared__ double redd[4][kWaves];
+    __shared__ int64_t redi[kWaves];
+    __shared__ double tot[4];
+    __shared__ int64_t toti;
+    __shared__ double pw[3];
+    // every thread sums a fixed strided subset of each partial array; the loads
+    // of all arrays are issued together (one memory round trip, not five)
     double h = 0, e = 0, pa = 0, pb = 0;
     int64_t o = 0;
-    if (ops & SC_HPWL) h = strided_sum(a.hpwl_part, a.n_hpwl);
-    if (ops & SC_POT) {
-        o = strided_sum(a.ovf_part, a.n_ovf);
-        e = strided_sum(a.energy_part, a.n_energy);
+    const int nh = (ops & SC_HPWL) ? a.n_hpwl : 0;
+    const int no = (ops & SC_POT) ? a.n_ovf : 0;
+    const int ne = (ops & SC_POT) ? a.n_energy : 0;
+    const int nc = (ops & (SC_L1 | SC_NEST)) ? a.n_cell : 0;
+    const int most = max(max(nh, no), max(ne, nc));
+    for (int k = threadIdx.x; k < most; k += kScalarThreads) {
+        const double vh = k < nh ? a.hpwl_part[k] : 0.0;
+        const int64_t vo = k < no ? a.ovf_part[k] : 0;
+        const double ve = k < ne ? a.energy_part[k] : 0.0;
+        const double va = k < nc ? a.part_a[k] : 0.0;
+        const double vb = k < nc ? a.part_b[k] : 0.0;
+        h += vh; o += vo; e += ve; pa += va; pb += vb;
     }
-    if (ops & (SC_L1 | SC_NEST)) {
-        pa = strided_sum(a.part_a, a.n_cell);
-        pb = strided_sum(a.part_b, a.n_cell);
+    // fixed-order reduction: lanes of a wavefront by butterfly, then the waves in order
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    h = wave_sum_f64(h); e = wave_sum_f64(e); pa = wave_sum_f64(pa); pb = wave_sum_f64(pb);
+    o = wave_sum_i64(o);
+    if (lane == 0) {
+        redd[0][wave] = h; redd[1][wave] = e; redd[2][wave] = pa; redd[3][wave] = pb;
+        redi[wave] = o;
     }
-    redd[0][threadIdx.x] = h;
-    redd[1][threadIdx.x] = e;
-    redd[2][threadIdx.x] = pa;
-    redd[3][threadIdx.x] = pb;
-    redi[threadIdx.x] = o;
     __syncthreads();
-    for (int st = kScalarThreads / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) redd[q][threadIdx.x] += redd[q][threadIdx.x + st];
-            redi[threadIdx.x] += redi[threadIdx.x + st];
-        }
-        __syncthreads();
+    if (threadIdx.x < 4) {
+        double v = 0.0;
+        for (int w = 0; w < kWaves; ++w) v += redd[threadIdx.x][w];
+        tot[threadIdx.x] = v;
+    } else if (threadIdx.x == 4) {
+        int64_t v = 0;
+        for (int w = 0; w < kWaves; ++w) v += redi[w];
+        toti = v;
     }
-    const double hsum = redd[0][0], esum = redd[1][0], asum = redd[2][0], bsum = redd[3][0];
-    const int64_t osum = redi[0];
+    __syncthreads();
+    const double hsum = tot[0], esum = tot[1], asum = tot[2], bsum = tot[3];
+    const int64_t osum = toti;
+    DevState *sp = a.st;
+    // scheduler.rs:92-130 needs three powers; lane 0 of waves 1..3 evaluates them
+    // concurrently from the values the update below will see.
+    if ((ops & SC_SCHED) && lane == 0 && wave >= 1 && wave <= 3) {
+        const DevState &s = *sp;
+        const double hcur = (ops & SC_HPWL) ? hsum : s.hpwl;
+        const double ocur = (ops & SC_POT) ? (double)osum * a.inv_scale_over_area : s.overflow;
+        if (wave == 1) {
+            pw[0] = pow(0.9999, (double)(int32_t)(s.iteration + 1));
+        } else if (wave == 2) {
+            const double init = s.hpwl_has_initial ? s.hpwl_initial : hcur;
+            pw[1] = pow(1.05, -(hcur - s.hpwl_cur) / init);
+        } else {
+            pw[2] = pow(10.0, (ocur - 0.1) * 20.0 / 9.0 - 1.0);
+        }
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
-    DevState &s = *a.st;
+    DevState &s = *sp;
     if (ops & SC_HPWL) s.hpwl = hsum;
     if (ops & SC_POT) {
         s.overflow = (double)osum * a.inv_scale_over_area;
@@ -281,18 +309,17 @@ __global__ void __launch_bounds__(kScalarThreads) k_scalar(const JobDesc *__rest
         if (!s.ovf_has_initial) { s.ovf_has_initial = 1; s.ovf_initial = s.overflow; }
         s.ovf_prev = s.ovf_cur;
         s.ovf_cur = s.overflow;
-        const double ref = s.hpwl_has_initial ? s.hpwl_initial : 350000.0;
         const double delta = s.hpwl_cur - s.hpwl_prev;
         double power;
         if (delta < 0.0) {
-            power = pow(0.9999, (double)(int32_t)s.iteration);
+            power = pw[0];
             power = power > 0.98 ? power : 0.98;
         } else {
-            power = pow(1.05, -delta / ref);
+            power = pw[1];
             power = power < 0.95 ? 0.95 : (power > 1.05 ? 1.05 : power);
         }
         s.density_weight = s.density_weight * (1.05 * power);
-        double coef = pow(10.0, (s.ovf_cur - 0.1) * 20.0 / 9.0 - 1.0);
+        double coef = pw[2];
         coef = coef < 0.05 ? 0.05 : (coef > 10.0 ? 10.0 : coef);
         s.gamma = s.base_gamma * coef;
         s.should_stop = s.hpwl_prev != 0.0 && s.hpwl_cur != 0.0 && s.hpwl_prev > s.hpwl_cur * 1.5;
