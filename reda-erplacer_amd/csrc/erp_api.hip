@@ -260,7 +260,7 @@ size_t erp::ctx_device_bytes(const erp_ctx *c) {
                             &c->prev_pos, &c->prev_g, &c->g, &c->gwl, &c->gd, &c->csize, &c->coff,
                             &c->ratio, &c->area, &c->npins, &c->hi, &c->slot_node, &c->slot_off,
                             &c->net_start, &c->slot_grad, &c->cstart, &c->cslots, &c->q_mov,
-                            &c->q_fix, &c->rho, &c->rowt, &c->p1, &c->p2, &c->p3, &c->fx, &c->fy,
+                            &c->q_fix, &c->rho, &c->rowt, &c->p1, &c->p2, &c->p3, &c->fxy,
                             &c->pot, &c->auv, &c->hpwl_part, &c->ovf_part, &c->energy_part,
                             &c->part_a, &c->part_b, &c->state, &c->gamma_override, &c->hpwl_dbu,
                             &c->keys, &c->perm, &c->perm_out, &c->hist, &c->cursor, &c->rec,
@@ -443,9 +443,9 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     TRY(ctx->slot_grad.alloc(f2 * (size_t)std::max<int64_t>(P, 1)));
     TRY(ctx->q_mov.alloc(sizeof(int64_t) * (size_t)ctx->B));
     TRY(ctx->q_fix.alloc(sizeof(int64_t) * (size_t)ctx->B));
-    for (DevBuf *b : {&ctx->rho, &ctx->rowt, &ctx->p1, &ctx->p2, &ctx->p3, &ctx->fx, &ctx->fy,
-                      &ctx->pot, &ctx->auv})
+    for (DevBuf *b : {&ctx->rho, &ctx->rowt, &ctx->p1, &ctx->p2, &ctx->p3, &ctx->pot, &ctx->auv})
         TRY(b->alloc(sizeof(float) * (size_t)ctx->B));
+    TRY(ctx->fxy.alloc(sizeof(float2) * (size_t)ctx->B));
     TRY(ctx->state.alloc(sizeof(DevState)));
     TRY(ctx->gamma_override.alloc(sizeof(double)));
     TRY(ctx->hpwl_dbu.alloc(sizeof(unsigned long long)));
@@ -478,8 +478,7 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     pb.p1 = ctx->p1.as<float>();
     pb.p2 = ctx->p2.as<float>();
     pb.p3 = ctx->p3.as<float>();
-    pb.fx = ctx->fx.as<float>();
-    pb.fy = ctx->fy.as<float>();
+    pb.fxy = ctx->fxy.as<float2>();
     pb.pot = ctx->pot.as<float>();
     pb.auv = ctx->auv.as<float>();
     pb.ovf_part = ctx->ovf_part.as<int64_t>();
@@ -520,8 +519,7 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     cb.pos_out = ctx->pos.as<float2>();
     cb.part_a = ctx->part_a.as<double>();
     cb.part_b = ctx->part_b.as<double>();
-    cb.fx = ctx->fx.as<float>();
-    cb.fy = ctx->fy.as<float>();
+    cb.fxy = ctx->fxy.as<float2>();
     cb.m = ctx->m;
     cb.n = ctx->n;
     cb.bw = ctx->bw;
@@ -795,8 +793,13 @@ int erp_get_map(erp_ctx *ctx, int which, float *out) {
         for (int64_t b = 0; b < ctx->B; ++b) out[b] = (float)((double)q[b] / ctx->fx_scale);
         return ERP_OK;
     }
+    if (which == ERP_MAP_FIELD_X || which == ERP_MAP_FIELD_Y) {   // one float of each pair
+        const char *base = ctx->fxy.as<char>() + (which == ERP_MAP_FIELD_Y ? sizeof(float) : 0);
+        ERP_HIP_TRY(hipMemcpy2D(out, sizeof(float), base, sizeof(float2), sizeof(float),
+                                (size_t)ctx->B, hipMemcpyDeviceToHost));
+        return ERP_OK;
+    }
     const DevBuf *src = which == ERP_MAP_DENSITY ? &ctx->rho : which == ERP_MAP_AUV ? &ctx->auv
-                      : which == ERP_MAP_FIELD_X ? &ctx->fx : which == ERP_MAP_FIELD_Y ? &ctx->fy
                       : which == ERP_MAP_POTENTIAL ? &ctx->pot : nullptr;
     if (!src) return fail(ERP_ERR_INVALID, "unknown map");
     ERP_HIP_TRY(hipMemcpy(out, src->p, sizeof(float) * (size_t)ctx->B, hipMemcpyDeviceToHost));
@@ -1100,12 +1103,13 @@ int erp_compute_electric_force(int64_t nc, int32_t m, int32_t n, const float *fi
     pack2(p, x, y, nc);
     pack2(sz, cw, ch, nc);
     pack2(of, ox, oy, nc);
-    DevBuf dp, ds, dof, dr, dfx, dfy, dg;
+    std::vector<float2> fxy;
+    pack2(fxy, field_x, field_y, (int64_t)B);
+    DevBuf dp, ds, dof, dr, dfxy, dg;
     int rc;
     if ((rc = upload(dp, p.data(), p.size(), L.s)) || (rc = upload(ds, sz.data(), sz.size(), L.s)) ||
         (rc = upload(dof, of.data(), of.size(), L.s)) || (rc = upload(dr, ratio, (size_t)nc, L.s)) ||
-        (rc = upload(dfx, field_x, B, L.s)) || (rc = upload(dfy, field_y, B, L.s)) ||
-        (rc = dg.alloc(sizeof(float2) * (size_t)nc)))
+        (rc = upload(dfxy, fxy.data(), fxy.size(), L.s)) || (rc = dg.alloc(sizeof(float2) * (size_t)nc)))
         return rc;
     CellBufs c{};
     c.C = nc;
@@ -1114,8 +1118,7 @@ int erp_compute_electric_force(int64_t nc, int32_t m, int32_t n, const float *fi
     c.coff = dof.as<float2>();
     c.ratio = dr.as<float>();
     c.gd = dg.as<float2>();
-    c.fx = dfx.as<float>();
-    c.fy = dfy.as<float>();
+    c.fxy = dfxy.as<float2>();
     c.m = m;
     c.n = n;
     c.bw = bw;

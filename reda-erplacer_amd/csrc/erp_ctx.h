@@ -24,7 +24,7 @@ struct erp_ctx {
     DevBuf pos, prev_pos, prev_g, g, gwl, gd;
     DevBuf csize, coff, ratio, area, npins, hi;
     DevBuf slot_node, slot_off, net_start, slot_grad, cstart, cslots;
-    DevBuf q_mov, q_fix, rho, rowt, p1, p2, p3, fx, fy, pot, auv;
+    DevBuf q_mov, q_fix, rho, rowt, p1, p2, p3, fxy, pot, auv;
     DevBuf hpwl_part, ovf_part, energy_part, part_a, part_b;
     DevBuf state, gamma_override, hpwl_dbu;
     DevBuf keys, perm, perm_out, hist, cursor, rec, rratio, sstat, sratio, sll, roff;

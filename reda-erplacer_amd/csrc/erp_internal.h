@@ -106,7 +106,8 @@ struct PotentialBufs {
     float *rho;             // [B]  density map after 1/bin_area scaling
     float *rowt;            // [B]  row-DCT intermediate
     float *p1, *p2, *p3;    // [B]  column-inverse intermediates
-    float *fx, *fy, *pot;   // [B]
+    float2 *fxy;            // [B]  field (x, y) per bin
+    float *pot;             // [B]
     float *auv;             // [B] or nullptr (store DCT coefficients)
     int64_t *ovf_part;      // [m / rows-per-block] int64 overflow partials
     double *energy_part;    // [m / rows-per-block] FP64 energy partials

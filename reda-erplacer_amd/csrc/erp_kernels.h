@@ -80,7 +80,7 @@ struct CellBufs {
     float2 *prev_pos, *prev_g;  // [C]
     float2 *pos_out;            // == pos (movable part written)
     double *part_a, *part_b;    // per-workgroup FP64 partials
-    const float *fx, *fy;       // field maps
+    const float2 *fxy;          // field maps, (x, y) interleaved per bin
     int m, n;
     float bw, bh;
     DevState *st;               // scalar state (density weight, alpha, beta)

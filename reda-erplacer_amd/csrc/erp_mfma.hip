@@ -51,6 +51,7 @@ constexpr int kStage = 4 * kAPlane + 4 * kBPlane;   // planes: [product e/o][hi/
 struct MfmaArgs {
     const float *src[3];
     float *dst[3];
+    int dst_stride[3];        // INV_ROW: element stride of each output (2 for the field pair)
     const __bf16 *tab;        // operator planes of the transformed length
     int H;                    // half length (the K of every product)
     int m, n;
@@ -329,11 +330,12 @@ __global__ void __launch_bounds__(256) k_mfma(MfmaArgs a) {
                 d[(size_t)row * n + col] = pe + po;
                 d[(size_t)(m - 1 - row) * n + col] = sine ? po - pe : pe - po;
             } else {
-                float *d = a.dst[z] + (size_t)row * n;
+                const int st = a.dst_stride[z];
+                float *d = a.dst[z] + (size_t)row * n * st;
                 const float x0 = 4.f * (pe + po);
                 const float x1 = 4.f * (sine ? po - pe : pe - po);
-                d[col] = x0;
-                d[n - 1 - col] = x1;
+                d[(size_t)col * st] = x0;
+                d[(size_t)(n - 1 - col) * st] = x1;
                 if (a.energy_part && z == 2) {
                     const float *rr = a.rho + (size_t)row * n;
                     energy += (double)(x0 * rr[col]) + (double)(x1 * rr[n - 1 - col]);
@@ -517,9 +519,9 @@ int forward(const MfmaPlan *p, const float *x, float *T, const MfmaArgs &colargs
 }
 
 // de-interleaved spectra X[z] -> column inverses Y[z] -> row inverses out[z].
-int inverse(const MfmaPlan *p, const float *const *X, float *const *Y, float *const *out, int nz,
-            int col_sine, int row_sine, const float *rho, double *energy_part, hipStream_t s,
-            KernelTimer *t) {
+int inverse(const MfmaPlan *p, const float *const *X, float *const *Y, float *const *out,
+            const int *out_stride, int nz, int col_sine, int row_sine, const float *rho,
+            double *energy_part, hipStream_t s, KernelTimer *t) {
     MfmaArgs a = base_args(p, p->m, p->n);
     for (int z = 0; z < nz; ++z) { a.src[z] = X[z]; a.dst[z] = Y[z]; }
     a.tab = p->tm();
@@ -530,7 +532,7 @@ int inverse(const MfmaPlan *p, const float *const *X, float *const *Y, float *co
     if (t) t->end();
     if (rc) return rc;
     MfmaArgs b = base_args(p, p->m, p->n);
-    for (int z = 0; z < nz; ++z) { b.src[z] = Y[z]; b.dst[z] = out[z]; }
+    for (int z = 0; z < nz; ++z) { b.src[z] = Y[z]; b.dst[z] = out[z]; b.dst_stride[z] = out_stride ? out_stride[z] : 1; }
     b.tab = p->tn();
     b.H = p->n / 2;
     b.sine_mask = row_sine;
@@ -566,9 +568,11 @@ int launch_potential_mfma(const PotentialBufs &b, MfmaPlan *p, hipStream_t s, Ke
     if (rc) return rc;
     const float *X[3] = {b.p1, b.p2, b.p3};
     float *Y[3] = {p->y0.as<float>(), p->y1.as<float>(), p->y2.as<float>()};
-    float *out[3] = {b.fx, b.fy, b.pot};
+    float *fxy = reinterpret_cast<float *>(b.fxy);
+    float *out[3] = {fxy, fxy + 1, b.pot};
+    const int stride[3] = {2, 2, 1};
     // xi_x: sine along x, cosine along y; xi_y: cosine along x, sine along y; phi: cosines
-    return inverse(p, X, Y, out, 3, /*col_sine=*/1, /*row_sine=*/2, b.rho, b.energy_part, s, t);
+    return inverse(p, X, Y, out, stride, 3, /*col_sine=*/1, /*row_sine=*/2, b.rho, b.energy_part, s, t);
 }
 
 // Leaf transforms (host-facing entry points run these on device buffers).
@@ -591,7 +595,8 @@ int launch_inverse_mfma(const float *x, float *out, int which, MfmaPlan *p, hipS
     const float *X[1] = {p->t0.as<float>()};
     float *Y[1] = {p->y0.as<float>()};
     float *O[1] = {out};
-    return inverse(p, X, Y, O, 1, which == 2 ? 1 : 0, which == 1 ? 1 : 0, nullptr, nullptr, s, nullptr);
+    return inverse(p, X, Y, O, nullptr, 1, which == 2 ? 1 : 0, which == 1 ? 1 : 0, nullptr, nullptr, s,
+                   nullptr);
 }
 
 }  // namespace erp

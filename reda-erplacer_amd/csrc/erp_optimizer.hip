@@ -7,14 +7,16 @@
 namespace erp {
 namespace {
 
+// Workgroup sum in a fixed order: butterfly inside each wavefront, then the
+// wavefront results in order.  red: kBlock / 64 doubles of LDS.
 __device__ __forceinline__ double block_sum_f64(double v, double *red) {
-    red[threadIdx.x] = v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    const double r = red[0];
+    double r = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) r += red[w];
     __syncthreads();
     return r;
 }
@@ -40,26 +42,36 @@ __device__ __forceinline__ float2 efield_cell(const CellBufs &c, int64_t i, floa
     int yh = (int)((ny + sz.y - 0.f) * inv_bh) + 1;
     yh = yh > c.n ? c.n : yh;
     float gx = 0.f, gy = 0.f;
-    for (int k = xl; k < xh; ++k) {
-        const float px = tri_unclamped(nx, sz.x, k, c.bw);
-        const float *fxr = c.fx + (size_t)k * c.n;
-        const float *fyr = c.fy + (size_t)k * c.n;
-        int h = yl;
-        for (; h + 4 <= yh; h += 4) {          // issue the row's loads together
-            float vx[4], vy[4];
+    if (yh - yl <= 4 && c.n >= 8) {
+        // a window of up to 4 bins lies in the 6 bins from the even bin at or
+        // below yl: three aligned 16-byte loads per row (both field components)
+        const int ys = min(yl & ~1, c.n - 6);
+        for (int k = xl; k < xh; ++k) {
+            const float px = tri_unclamped(nx, sz.x, k, c.bw);
+            const float4 *row = reinterpret_cast<const float4 *>(c.fxy + (size_t)k * c.n + ys);
+            const float4 v0 = row[0], v1 = row[1], v2 = row[2];
+            const float vx[6] = {v0.x, v0.z, v1.x, v1.z, v2.x, v2.z};
+            const float vy[6] = {v0.y, v0.w, v1.y, v1.w, v2.y, v2.w};
 #pragma unroll
-            for (int j = 0; j < 4; ++j) { vx[j] = fxr[h + j]; vy[j] = fyr[h + j]; }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float area = px * tri_unclamped(ny, sz.y, h + j, c.bh);
-                gx += area * vx[j];
-                gy += area * vy[j];
+            for (int j = 0; j < 6; ++j) {
+                const int h = ys + j;
+                if (h >= yl && h < yh) {
+                    const float area = px * tri_unclamped(ny, sz.y, h, c.bh);
+                    gx += area * vx[j];
+                    gy += area * vy[j];
+                }
             }
         }
-        for (; h < yh; ++h) {
-            const float area = px * tri_unclamped(ny, sz.y, h, c.bh);
-            gx += area * fxr[h];
-            gy += area * fyr[h];
+    } else {
+        for (int k = xl; k < xh; ++k) {
+            const float px = tri_unclamped(nx, sz.x, k, c.bw);
+            const float2 *row = c.fxy + (size_t)k * c.n;
+            for (int h = yl; h < yh; ++h) {
+                const float2 v = row[h];
+                const float area = px * tri_unclamped(ny, sz.y, h, c.bh);
+                gx += area * v.x;
+                gy += area * v.y;
+            }
         }
     }
     const float r = c.ratio[i];
@@ -112,7 +124,7 @@ __global__ void __launch_bounds__(kBlock) k_efield(const JobDesc *__restrict__ j
 }
 
 __global__ void __launch_bounds__(kBlock) k_l1(const JobDesc *__restrict__ jd) {
-    __shared__ double red[kBlock];
+    __shared__ double red[kBlock / 64];
     ERP_CELL_JOB();
     double a = 0.0, b = 0.0;
     if (i < c.C) {
@@ -139,7 +151,7 @@ __device__ __forceinline__ void bb_terms(const CellBufs &c, int64_t i, float2 g,
 }
 
 __global__ void __launch_bounds__(kBlock) k_combine(const JobDesc *__restrict__ jd) {
-    __shared__ double red[kBlock];
+    __shared__ double red[kBlock / 64];
     ERP_CELL_JOB();
     const DevState *__restrict__ st = c.st;
     double a = 0.0, b = 0.0;
@@ -158,7 +170,7 @@ __global__ void __launch_bounds__(kBlock) k_combine(const JobDesc *__restrict__ 
 
 template <bool STORE_PARTS>
 __global__ void __launch_bounds__(kBlock) k_cell_fused(const JobDesc *__restrict__ jd) {
-    __shared__ double red[kBlock];
+    __shared__ double red[kBlock / 64];
     ERP_CELL_JOB();
     const DevState *__restrict__ st = c.st;
     double a = 0.0, b = 0.0;

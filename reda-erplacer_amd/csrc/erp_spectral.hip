@@ -400,8 +400,7 @@ __global__ void __launch_bounds__(kBlock) k_row_final(const JobDesc *__restrict_
     const float *__restrict__ P2 = pb.p2;
     const float *__restrict__ P3 = pb.p3;
     const float *__restrict__ rho = pb.rho;
-    float *__restrict__ fx = pb.fx;
-    float *__restrict__ fy = pb.fy;
+    float2 *__restrict__ fxy = pb.fxy;
     float *__restrict__ pot = pb.pot;
     double *__restrict__ energy_part = pb.energy_part;
     const int n = pb.n;
@@ -433,10 +432,8 @@ __global__ void __launch_bounds__(kBlock) k_row_final(const JobDesc *__restrict_
         const float2 vx = buf[pos], vy = buf[pos + (F << logN)], vp = buf[pos + (2 * F << logN)];
         const float sg = (j & 1) ? -1.f : 1.f;
         const size_t o0 = (size_t)(row_base + 2 * f) * n + j;
-        fx[o0] = vx.x;
-        fx[o0 + n] = vx.y;
-        fy[o0] = sg * vy.x;
-        fy[o0 + n] = sg * vy.y;
+        fxy[o0] = make_float2(vx.x, sg * vy.x);
+        fxy[o0 + n] = make_float2(vx.y, sg * vy.y);
         pot[o0] = vp.x;
         pot[o0 + n] = vp.y;
         e += (double)(vp.x * rho[o0]) + (double)(vp.y * rho[o0 + n]);
