@@ -201,16 +201,25 @@ __global__ void __launch_bounds__(kBlock) k_sort_keys(const JobDesc *__restrict_
     for (int t = threadIdx.x; t < ntiles; t += kBlock) lh[t] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kScatterCells;
+    // all loads of the thread's cells first (they are then in flight together)
+    int32_t cid[kScatterPer];
+    float4 stv[kScatterPer];
+    float2 pv[kScatterPer];
+#pragma unroll
+    for (int j = 0; j < kScatterPer; ++j) {
+        const int64_t i = base + threadIdx.x + (int64_t)j * kBlock;
+        cid[j] = i < num_cells ? perm[i] : -1;
+        stv[j] = i < num_cells ? sstat[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int j = 0; j < kScatterPer; ++j) pv[j] = cid[j] >= 0 ? pos[cid[j]] : make_float2(0.f, 0.f);
 #pragma unroll
     for (int j = 0; j < kScatterPer; ++j) {
         const int64_t i = base + threadIdx.x + (int64_t)j * kBlock;
         const bool valid = i < num_cells;
         int k = -1;
         if (valid) {
-            const int32_t c = perm[i];
-            const float2 p = pos[c];
-            const float4 st = sstat[i];
-            const float2 l = make_float2(p.x + st.z, p.y + st.w);
+            const float2 l = make_float2(pv[j].x + stv[j].z, pv[j].y + stv[j].w);
             k = sort_key(l.x, l.y, inv_bw, inv_bh, m, n, shift);
             keys[i] = k;
             ll[i] = l;
@@ -283,11 +292,25 @@ __global__ void __launch_bounds__(kBlock) k_sort_place(const JobDesc *__restrict
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kScatterCells;
     int key[kScatterPer], rank[kScatterPer];
+    int32_t cperm[kScatterPer];
+    float2 cll[kScatterPer];
+    float4 cst[kScatterPer];
+    float crat[kScatterPer];
+    // the thread's records are loaded up front, all in flight together
 #pragma unroll
     for (int j = 0; j < kScatterPer; ++j) {
         const int64_t i = base + threadIdx.x + (int64_t)j * kBlock;
         const bool valid = i < num_cells;
         key[j] = valid ? keys[i] : -1;
+        cperm[j] = valid ? perm[i] : 0;
+        cll[j] = valid ? ll[i] : make_float2(0.f, 0.f);
+        cst[j] = valid ? sstat[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        crat[j] = valid ? sratio[i] : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < kScatterPer; ++j) {
+        const int64_t i = base + threadIdx.x + (int64_t)j * kBlock;
+        const bool valid = i < num_cells;
         rank[j] = 0;
         // valid lanes are a prefix of the wavefront; if they share one tile the
         // wavefront reserves its ranks with one LDS atomic
@@ -308,14 +331,11 @@ __global__ void __launch_bounds__(kBlock) k_sort_place(const JobDesc *__restrict
 #pragma unroll
     for (int j = 0; j < kScatterPer; ++j) {
         if (key[j] >= 0) {
-            const int64_t i = base + threadIdx.x + (int64_t)j * kBlock;
             const int dst = lh[key[j]] + rank[j];
-            const float2 l = ll[i];
-            const float4 st = sstat[i];
-            perm_out[dst] = perm[i];
-            rec[dst] = make_float4(l.x, l.y, st.x, st.y);
-            rratio[dst] = sratio[i];
-            roff[dst] = make_float2(st.z, st.w);
+            perm_out[dst] = cperm[j];
+            rec[dst] = make_float4(cll[j].x, cll[j].y, cst[j].x, cst[j].y);
+            rratio[dst] = crat[j];
+            roff[dst] = make_float2(cst[j].z, cst[j].w);
         }
     }
 }
