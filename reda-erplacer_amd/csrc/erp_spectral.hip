@@ -144,27 +144,51 @@ __device__ __forceinline__ void row_dct_body(
     load_tw(tw, fft_tw, n >> 1);
     const int row_base = blockIdx.x * 2 * F;
     int64_t ovf = 0;
-    for (int idx = threadIdx.x; idx < (F << logN); idx += blockDim.x) {
-        const int f = idx >> logN, c = idx & (n - 1);
-        const size_t o0 = (size_t)(row_base + 2 * f) * n + c, o1 = o0 + n;
-        float a, b;
-        if (SRC_FIXED) {
-            const int64_t t0 = q_mov[o0] + q_fix[o0];
-            const int64_t t1 = q_mov[o1] + q_fix[o1];
-            if (clear_q) {
-                q_mov[o0] = 0;
-                q_mov[o1] = 0;
+    // chunks of kCh elements per thread: every load of a chunk is issued
+    // before any is used
+    constexpr int kCh = 4;
+    const int total = F << logN;
+    for (int idx0 = threadIdx.x; idx0 < total; idx0 += kCh * blockDim.x) {
+        int64_t t0[kCh], t1[kCh];
+        float xa[kCh], xb[kCh];
+#pragma unroll
+        for (int r = 0; r < kCh; ++r) {
+            const int idx = idx0 + r * blockDim.x;
+            if (idx < total) {
+                const int f = idx >> logN, c = idx & (n - 1);
+                const size_t o0 = (size_t)(row_base + 2 * f) * n + c, o1 = o0 + n;
+                if (SRC_FIXED) {
+                    t0[r] = q_mov[o0] + q_fix[o0];
+                    t1[r] = q_mov[o1] + q_fix[o1];
+                } else {
+                    xa[r] = x[o0];
+                    xb[r] = x[o1];
+                }
             }
-            ovf += (t0 > coef_q ? t0 - coef_q : 0) + (t1 > coef_q ? t1 - coef_q : 0);
-            a = (float)((double)t0 * inv_scale) * inv_bin_area;
-            b = (float)((double)t1 * inv_scale) * inv_bin_area;
-            rho_out[o0] = a;
-            rho_out[o1] = b;
-        } else {
-            a = x[o0];
-            b = x[o1];
         }
-        buf[(f << logN) + brev(perm(c, n), logN)] = make_float2(a, b);
+#pragma unroll
+        for (int r = 0; r < kCh; ++r) {
+            const int idx = idx0 + r * blockDim.x;
+            if (idx >= total) continue;
+            const int f = idx >> logN, c = idx & (n - 1);
+            const size_t o0 = (size_t)(row_base + 2 * f) * n + c, o1 = o0 + n;
+            float a, b;
+            if (SRC_FIXED) {
+                if (clear_q) {
+                    q_mov[o0] = 0;
+                    q_mov[o1] = 0;
+                }
+                ovf += (t0[r] > coef_q ? t0[r] - coef_q : 0) + (t1[r] > coef_q ? t1[r] - coef_q : 0);
+                a = (float)((double)t0[r] * inv_scale) * inv_bin_area;
+                b = (float)((double)t1[r] * inv_scale) * inv_bin_area;
+                rho_out[o0] = a;
+                rho_out[o1] = b;
+            } else {
+                a = xa[r];
+                b = xb[r];
+            }
+            buf[(f << logN) + brev(perm(c, n), logN)] = make_float2(a, b);
+        }
     }
     if (SRC_FIXED) {
         __shared__ int64_t red[kBlock];
@@ -325,13 +349,29 @@ __global__ void __launch_bounds__(kBlock) k_col_spectral(const JobDesc *__restri
     float2 *tw = lds;
     float2 *buf = lds + (m >> 1);                          // 3G complex arrays of m
     float *acol = reinterpret_cast<float *>(buf + 3 * G * m);  // S real arrays of m
-    load_tw(tw, fft_tw, m >> 1);
     const int v0 = strip_of_block(blockIdx.x, gridDim.x) * S;
-    for (int idx = threadIdx.x; idx < m * S; idx += blockDim.x) {
-        const int i = idx / S, s = idx - i * S;
-        const float val = T[(size_t)i * n + v0 + s];
-        float *dst = reinterpret_cast<float *>(&buf[((s >> 1) << logM) + brev(perm(i, m), logM)]);
-        dst[s & 1] = val;
+    // the strip's column segments are loaded into registers first (all loads
+    // in flight together: one workgroup per CU cannot hide them one by one)
+    constexpr int kMaxPer = 16;                      // m * S / blockDim <= 16 for m <= 2048
+    const int per = (m * S) / (int)blockDim.x;
+    float vals[kMaxPer];
+#pragma unroll
+    for (int r = 0; r < kMaxPer; ++r) {
+        if (r < per) {
+            const int idx = threadIdx.x + r * blockDim.x;
+            const int i = idx / S, s = idx - i * S;
+            vals[r] = T[(size_t)i * n + v0 + s];
+        }
+    }
+    load_tw(tw, fft_tw, m >> 1);
+#pragma unroll
+    for (int r = 0; r < kMaxPer; ++r) {
+        if (r < per) {
+            const int idx = threadIdx.x + r * blockDim.x;
+            const int i = idx / S, s = idx - i * S;
+            float *dst = reinterpret_cast<float *>(&buf[((s >> 1) << logM) + brev(perm(i, m), logM)]);
+            dst[s & 1] = vals[r];
+        }
     }
     fft_stages<false>(buf, G, logM, tw);
     for (int idx = threadIdx.x; idx < m * G; idx += blockDim.x) {
@@ -410,19 +450,37 @@ __global__ void __launch_bounds__(kBlock) k_row_final(const JobDesc *__restrict_
     float2 *buf = lds + (n >> 1);   // 3F arrays
     load_tw(tw, fft_tw, n >> 1);
     const int row_base = blockIdx.x * 2 * F;
-    for (int idx = threadIdx.x; idx < (F << logN); idx += blockDim.x) {
-        const int f = idx >> logN, k = idx & (n - 1);
-        const size_t r0 = (size_t)(row_base + 2 * f) * n;
-        const float2 w = dct_tw[k];
-        const int pos = (f << logN) + brev(k, logN);
-        float2 a = spec_pair<INV_IDCT>(P1 + r0, k, n, 1), b = spec_pair<INV_IDCT>(P1 + r0 + n, k, n, 1);
-        buf[pos] = pack2(inv_twiddle(a.x, a.y, w), inv_twiddle(b.x, b.y, w));
-        a = spec_pair<INV_IDXST>(P2 + r0, k, n, 1);
-        b = spec_pair<INV_IDXST>(P2 + r0 + n, k, n, 1);
-        buf[pos + (F << logN)] = pack2(inv_twiddle(a.x, a.y, w), inv_twiddle(b.x, b.y, w));
-        a = spec_pair<INV_IDCT>(P3 + r0, k, n, 1);
-        b = spec_pair<INV_IDCT>(P3 + r0 + n, k, n, 1);
-        buf[pos + (2 * F << logN)] = pack2(inv_twiddle(a.x, a.y, w), inv_twiddle(b.x, b.y, w));
+    constexpr int kCh = 4;      // elements per thread whose loads are issued together
+    const int total = F << logN;
+    for (int idx0 = threadIdx.x; idx0 < total; idx0 += kCh * blockDim.x) {
+        float2 q[kCh][6];
+#pragma unroll
+        for (int r = 0; r < kCh; ++r) {
+            const int idx = idx0 + r * blockDim.x;
+            if (idx < total) {
+                const int f = idx >> logN, k = idx & (n - 1);
+                const size_t r0 = (size_t)(row_base + 2 * f) * n;
+                q[r][0] = spec_pair<INV_IDCT>(P1 + r0, k, n, 1);
+                q[r][1] = spec_pair<INV_IDCT>(P1 + r0 + n, k, n, 1);
+                q[r][2] = spec_pair<INV_IDXST>(P2 + r0, k, n, 1);
+                q[r][3] = spec_pair<INV_IDXST>(P2 + r0 + n, k, n, 1);
+                q[r][4] = spec_pair<INV_IDCT>(P3 + r0, k, n, 1);
+                q[r][5] = spec_pair<INV_IDCT>(P3 + r0 + n, k, n, 1);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kCh; ++r) {
+            const int idx = idx0 + r * blockDim.x;
+            if (idx >= total) continue;
+            const int f = idx >> logN, k = idx & (n - 1);
+            const float2 w = dct_tw[k];
+            const int pos = (f << logN) + brev(k, logN);
+            buf[pos] = pack2(inv_twiddle(q[r][0].x, q[r][0].y, w), inv_twiddle(q[r][1].x, q[r][1].y, w));
+            buf[pos + (F << logN)] = pack2(inv_twiddle(q[r][2].x, q[r][2].y, w),
+                                           inv_twiddle(q[r][3].x, q[r][3].y, w));
+            buf[pos + (2 * F << logN)] = pack2(inv_twiddle(q[r][4].x, q[r][4].y, w),
+                                               inv_twiddle(q[r][5].x, q[r][5].y, w));
+        }
     }
     fft_stages<true>(buf, 3 * F, logN, tw);
     double e = 0.0;
