@@ -140,8 +140,11 @@ __device__ __forceinline__ void row_dct_body(
     double inv_scale, int64_t coef_q, float inv_bin_area, bool clear_q) {
     extern __shared__ float2 lds[];
     float2 *tw = lds;
-    float2 *buf = lds + (n >> 1);
+    float2 *dtw_s = lds + (n >> 1);   // e^{-i pi k / 2L} staged in LDS
+    float2 *buf = dtw_s + n;
     load_tw(tw, fft_tw, n >> 1);
+    load_tw(dtw_s, dct_tw, n);
+    __syncthreads();
     const int row_base = blockIdx.x * 2 * F;
     int64_t ovf = 0;
     // chunks of kCh elements per thread: every load of a chunk is issued
@@ -203,7 +206,7 @@ __device__ __forceinline__ void row_dct_body(
     fft_stages<false>(buf, F, logN, tw);
     for (int idx = threadIdx.x; idx < (F << logN); idx += blockDim.x) {
         const int f = idx >> logN, k = idx & (n - 1);
-        const float2 y = dct_unpack(buf + (f << logN), k, n, dct_tw[k]);
+        const float2 y = dct_unpack(buf + (f << logN), k, n, dtw_s[k]);
         const size_t o0 = (size_t)(row_base + 2 * f) * n + k;
         T[o0] = 0.5f * y.x;
         T[o0 + n] = 0.5f * y.y;
@@ -244,8 +247,11 @@ __global__ void __launch_bounds__(kBlock) k_col_dct(const float *__restrict__ T,
                                                     float scale) {
     extern __shared__ float2 lds[];
     float2 *tw = lds;
-    float2 *buf = lds + (m >> 1);
+    float2 *dtw_s = lds + (m >> 1);   // e^{-i pi k / 2L} staged in LDS
+    float2 *buf = dtw_s + m;
     load_tw(tw, fft_tw, m >> 1);
+    load_tw(dtw_s, dct_tw, m);
+    __syncthreads();
     const int S = 2 * G;
     const int v0 = strip_of_block(blockIdx.x, gridDim.x) * S;
     for (int idx = threadIdx.x; idx < m * S; idx += blockDim.x) {
@@ -257,7 +263,7 @@ __global__ void __launch_bounds__(kBlock) k_col_dct(const float *__restrict__ T,
     fft_stages<false>(buf, G, logM, tw);
     for (int idx = threadIdx.x; idx < m * G; idx += blockDim.x) {
         const int u = idx / G, g = idx - u * G;
-        const float2 y = dct_unpack(buf + (g << logM), u, m, dct_tw[u]);
+        const float2 y = dct_unpack(buf + (g << logM), u, m, dtw_s[u]);
         out[(size_t)u * n + v0 + 2 * g] = 0.5f * scale * y.x;
         out[(size_t)u * n + v0 + 2 * g + 1] = 0.5f * scale * y.y;
     }
@@ -274,13 +280,16 @@ __global__ void __launch_bounds__(kBlock) k_col_inv(const float *__restrict__ X,
                                                     const float2 *__restrict__ dct_tw) {
     extern __shared__ float2 lds[];
     float2 *tw = lds;
-    float2 *buf = lds + (m >> 1);
+    float2 *dtw_s = lds + (m >> 1);   // e^{-i pi k / 2L} staged in LDS
+    float2 *buf = dtw_s + m;
     load_tw(tw, fft_tw, m >> 1);
+    load_tw(dtw_s, dct_tw, m);
+    __syncthreads();
     const int S = 2 * G;
     const int v0 = strip_of_block(blockIdx.x, gridDim.x) * S;
     for (int idx = threadIdx.x; idx < m * G; idx += blockDim.x) {
         const int k = idx / G, g = idx - k * G;
-        const float2 w = dct_tw[k];
+        const float2 w = dtw_s[k];
         const float2 p0 = spec_pair<KIND>(X + v0 + 2 * g, k, m, n);
         const float2 p1 = spec_pair<KIND>(X + v0 + 2 * g + 1, k, m, n);
         buf[(g << logM) + brev(k, logM)] = pack2(inv_twiddle(p0.x, p0.y, w), inv_twiddle(p1.x, p1.y, w));
@@ -304,13 +313,16 @@ __global__ void __launch_bounds__(kBlock) k_row_inv(const float *__restrict__ X,
                                                     const float2 *__restrict__ dct_tw) {
     extern __shared__ float2 lds[];
     float2 *tw = lds;
-    float2 *buf = lds + (n >> 1);
+    float2 *dtw_s = lds + (n >> 1);   // e^{-i pi k / 2L} staged in LDS
+    float2 *buf = dtw_s + n;
     load_tw(tw, fft_tw, n >> 1);
+    load_tw(dtw_s, dct_tw, n);
+    __syncthreads();
     const int row_base = blockIdx.x * 2 * F;
     for (int idx = threadIdx.x; idx < (F << logN); idx += blockDim.x) {
         const int f = idx >> logN, k = idx & (n - 1);
         const float *r0 = X + (size_t)(row_base + 2 * f) * n;
-        const float2 w = dct_tw[k];
+        const float2 w = dtw_s[k];
         const float2 p0 = spec_pair<KIND>(r0, k, n, 1);
         const float2 p1 = spec_pair<KIND>(r0 + n, k, n, 1);
         buf[(f << logN) + brev(k, logN)] = pack2(inv_twiddle(p0.x, p0.y, w), inv_twiddle(p1.x, p1.y, w));
@@ -347,27 +359,31 @@ __global__ void __launch_bounds__(kBlock) k_col_spectral(const JobDesc *__restri
     const float wv_div = pb.bw / pb.bh;
     const int S = 2 * G;
     float2 *tw = lds;
-    float2 *buf = lds + (m >> 1);                          // 3G complex arrays of m
+    float2 *dtw_s = lds + (m >> 1);   // e^{-i pi k / 2L} staged in LDS
+    float2 *buf = dtw_s + m;                          // 3G complex arrays of m
     float *acol = reinterpret_cast<float *>(buf + 3 * G * m);  // S real arrays of m
     const int v0 = strip_of_block(blockIdx.x, gridDim.x) * S;
     // the strip's column segments are loaded into registers first (all loads
     // in flight together: one workgroup per CU cannot hide them one by one)
     constexpr int kMaxPer = 16;                      // m * S / blockDim <= 16 for m <= 2048
-    const int per = (m * S) / (int)blockDim.x;
+    const int total = m * S;
+    const int per = (total + (int)blockDim.x - 1) / (int)blockDim.x;
     float vals[kMaxPer];
 #pragma unroll
     for (int r = 0; r < kMaxPer; ++r) {
-        if (r < per) {
-            const int idx = threadIdx.x + r * blockDim.x;
+        const int idx = threadIdx.x + r * blockDim.x;
+        if (r < per && idx < total) {
             const int i = idx / S, s = idx - i * S;
             vals[r] = T[(size_t)i * n + v0 + s];
         }
     }
     load_tw(tw, fft_tw, m >> 1);
+    load_tw(dtw_s, dct_tw, m);
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < kMaxPer; ++r) {
-        if (r < per) {
-            const int idx = threadIdx.x + r * blockDim.x;
+        const int idx = threadIdx.x + r * blockDim.x;
+        if (r < per && idx < total) {
             const int i = idx / S, s = idx - i * S;
             float *dst = reinterpret_cast<float *>(&buf[((s >> 1) << logM) + brev(perm(i, m), logM)]);
             dst[s & 1] = vals[r];
@@ -376,7 +392,7 @@ __global__ void __launch_bounds__(kBlock) k_col_spectral(const JobDesc *__restri
     fft_stages<false>(buf, G, logM, tw);
     for (int idx = threadIdx.x; idx < m * G; idx += blockDim.x) {
         const int u = idx / G, g = idx - u * G;
-        const float2 y = dct_unpack(buf + (g << logM), u, m, dct_tw[u]);
+        const float2 y = dct_unpack(buf + (g << logM), u, m, dtw_s[u]);
         const float a0 = 0.5f * scale * y.x, a1 = 0.5f * scale * y.y;
         acol[(2 * g) * m + u] = a0;
         acol[(2 * g + 1) * m + u] = a1;
@@ -390,7 +406,7 @@ __global__ void __launch_bounds__(kBlock) k_col_spectral(const JobDesc *__restri
     for (int idx = threadIdx.x; idx < m * G; idx += blockDim.x) {
         const int k = idx / G, g = idx - k * G;
         const int kr = (m - k) & (m - 1);
-        const float2 w = dct_tw[k];
+        const float2 w = dtw_s[k];
         float2 vx[2], vy[2], vp[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -447,8 +463,11 @@ __global__ void __launch_bounds__(kBlock) k_row_final(const JobDesc *__restrict_
     const float2 *__restrict__ fft_tw = pb.tn_fft;
     const float2 *__restrict__ dct_tw = pb.tn_dct;
     float2 *tw = lds;
-    float2 *buf = lds + (n >> 1);   // 3F arrays
+    float2 *dtw_s = lds + (n >> 1);   // e^{-i pi k / 2L} staged in LDS
+    float2 *buf = dtw_s + n;   // 3F arrays
     load_tw(tw, fft_tw, n >> 1);
+    load_tw(dtw_s, dct_tw, n);
+    __syncthreads();
     const int row_base = blockIdx.x * 2 * F;
     constexpr int kCh = 4;      // elements per thread whose loads are issued together
     const int total = F << logN;
@@ -473,7 +492,7 @@ __global__ void __launch_bounds__(kBlock) k_row_final(const JobDesc *__restrict_
             const int idx = idx0 + r * blockDim.x;
             if (idx >= total) continue;
             const int f = idx >> logN, k = idx & (n - 1);
-            const float2 w = dct_tw[k];
+            const float2 w = dtw_s[k];
             const int pos = (f << logN) + brev(k, logN);
             buf[pos] = pack2(inv_twiddle(q[r][0].x, q[r][0].y, w), inv_twiddle(q[r][1].x, q[r][1].y, w));
             buf[pos + (F << logN)] = pack2(inv_twiddle(q[r][2].x, q[r][2].y, w),
@@ -522,9 +541,9 @@ __global__ void k_make_tables(float2 *fft_tw, float2 *dct_tw, int L) {
 
 // rows per block of the row passes, columns per block of the column pass
 constexpr int kRowPairs = 2;   // F: packed pairs of rows per workgroup (4 rows)
-inline int col_pairs(int m) { return m >= 2048 ? 1 : 2; }
+inline int col_pairs(int m) { return 1; }   // one packed column pair per workgroup (more workgroups in flight)
 
-size_t row_lds(int n, int arrays) { return sizeof(float2) * ((size_t)n / 2 + (size_t)arrays * n); }
+size_t row_lds(int n, int arrays) { return sizeof(float2) * ((size_t)n / 2 + (size_t)n + (size_t)arrays * n); }
 
 // Raise a kernel's dynamic-LDS limit once per (device, kernel) to the largest size
 // requested so far, so that captured launches never need the attribute call.
@@ -618,7 +637,7 @@ int launch_potential_fft(const JobDesc *d, const BatchDims &bd, int nj, hipStrea
     // pass 2 (operator constants exactly as frequencies.rs computes them in T=f32)
     const float two_pi_over_m = (float)(2.0 * M_PI) / (float)m;
     const float two_pi_over_n = (float)(2.0 * M_PI) / (float)n;
-    size_t l2 = sizeof(float2) * ((size_t)m / 2 + (size_t)3 * G * m) + sizeof(float) * (size_t)2 * G * m;
+    size_t l2 = sizeof(float2) * ((size_t)m / 2 + (size_t)m + (size_t)3 * G * m) + sizeof(float) * (size_t)2 * G * m;
     if ((rc = set_lds_limit((const void *)k_col_spectral, l2))) return rc;
     if (t) t->begin("pass2_col_spectral");
     hipLaunchKernelGGL(k_col_spectral, dim3(n / (2 * G), nj), dim3(kBlock), l2, s, d, logM, G,
