@@ -540,7 +540,7 @@ __global__ void k_make_tables(float2 *fft_tw, float2 *dct_tw, int L) {
 }
 
 // rows per block of the row passes, columns per block of the column pass
-constexpr int kRowPairs = 2;   // F: packed pairs of rows per workgroup (4 rows)
+constexpr int kRowPairs = 1;   // F: packed pairs of rows per workgroup (2 rows)
 inline int col_pairs(int m) { return 1; }   // one packed column pair per workgroup (more workgroups in flight)
 
 size_t row_lds(int n, int arrays) { return sizeof(float2) * ((size_t)n / 2 + (size_t)n + (size_t)arrays * n); }
