@@ -72,6 +72,23 @@ def kernel_bytes(name, d, grid, cells_pins):
     return table.get(name, 0)
 
 
+# timer name -> kernel function (without the k_ prefix) in the rocprofv3 summaries
+KERNEL_OF_TIMER = {"pass1_row_dct": "pass1", "pass2_col_spectral": "col_spectral",
+                   "pass3_row_final": "row_final"}
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)["bytes_per_dispatch"]
+    v = table.get(KERNEL_OF_TIMER.get(kernel, kernel))
+    return int(round(v)) if v is not None else None
+
+
 def cpu_baseline(d, grid, td, iters, threads):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
@@ -259,7 +276,8 @@ def main():
     nbytes = kernel_bytes(dom, d, grid, pins_cells)
     achieved = nbytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(args.workload, dom) if args.spectral == "fft" else None,
             "bytes_per_launch": nbytes, "avg_launch_ms": round(dom_ms, 5)}
     kernels = {k: {"avg_ms": round(v, 5),
                    "gbs": round(kernel_bytes(k, d, grid, pins_cells) / (v * 1e-3) / 1e9, 1) if v > 0 else None}
