@@ -100,10 +100,11 @@ def cpu_baseline(d, grid, td, iters, threads):
     return iters / dt, dt
 
 
-def serving_bench(n_requests, rate, check_every, max_active, max_iterations):
+def serving_bench(n_requests, rates, check_every, max_active, max_iterations):
     """Serving form: the request stream (log-uniform 2k-150k cells, grids 64-512,
     stop at overflow < 0.1) placed (1) one after another alone, (2) all submitted at
-    once to the continuous-batching server, (3) with Poisson arrivals at `rate`.
+    once to the continuous-batching server, (3) with Poisson arrivals at each rate
+    (the request sizes are the same; only the arrival times scale).
     Returns the serving object of the JSON line."""
     import numpy as np
     from erplacer import workloads as W
@@ -112,7 +113,7 @@ def serving_bench(n_requests, rate, check_every, max_active, max_iterations):
     stop = W.SERVING["stop_overflow"]
     td = W.SERVING["target_density"]
     t0 = time.perf_counter()
-    reqs = W.serving_requests(n_requests, rate_per_s=rate)
+    reqs = W.serving_requests(n_requests, rate_per_s=rates[0])
     print(f"[serving] generated {len(reqs)} requests in {time.perf_counter() - t0:.1f}s "
           f"({sum(d.num_movable for _, d, _ in reqs)} cells)", file=sys.stderr, flush=True)
 
@@ -146,31 +147,40 @@ def serving_bench(n_requests, rate, check_every, max_active, max_iterations):
 
     res_sat, mk_sat, st_sat = served([0.0] * len(reqs))
     print(f"[serving] saturated: {mk_sat:.2f}s stats {st_sat}", file=sys.stderr, flush=True)
-    arrivals = [a for a, _, _ in reqs]
-    res_poi, mk_poi, st_poi = served(arrivals)
-    print(f"[serving] poisson {rate}/s: {mk_poi:.2f}s stats {st_poi}", file=sys.stderr, flush=True)
-
-    same = all(r["status"] == 0 and (r["iterations"], r["hpwl_dbu"]) == s
-               for res in (res_sat, res_poi) for r, s in zip(res, seq_res))
-    lat = np.array([r["t_done"] - r["t_submit"] for r in res_poi])
-    # sequential server under the same arrivals: FIFO, one solve at a time
-    done, seq_lat = 0.0, []
-    for a, t in zip(arrivals, seq_t):
-        done = max(done, a) + t
-        seq_lat.append(done - a)
-    seq_lat = np.array(seq_lat)
+    same = all(r["status"] == 0 and (r["iterations"], r["hpwl_dbu"]) == s for r, s in zip(res_sat, seq_res))
+    base_arrivals = np.array([a for a, _, _ in reqs])
+    latency = []
+    for rate in rates:
+        arrivals = list(base_arrivals * (rates[0] / rate))
+        res_poi, mk_poi, st_poi = served(arrivals)
+        same = same and all(r["status"] == 0 and (r["iterations"], r["hpwl_dbu"]) == s
+                            for r, s in zip(res_poi, seq_res))
+        lat = np.array([r["t_done"] - r["t_submit"] for r in res_poi])
+        # a sequential server under the same arrivals: FIFO, one alone solve at a time
+        done, seq_lat = 0.0, []
+        for a, t in zip(arrivals, seq_t):
+            done = max(done, a) + t
+            seq_lat.append(done - a)
+        seq_lat = np.array(seq_lat)
+        row = {"arrival_rate_per_s": rate,
+               "p50_s": round(float(np.percentile(lat, 50)), 4),
+               "p99_s": round(float(np.percentile(lat, 99)), 4),
+               "sequential_p50_s": round(float(np.percentile(seq_lat, 50)), 4),
+               "sequential_p99_s": round(float(np.percentile(seq_lat, 99)), 4),
+               "makespan_s": round(mk_poi, 3), "max_batch": st_poi["max_batch"]}
+        print(f"[serving] poisson {rate}/s: {row}", file=sys.stderr, flush=True)
+        latency.append(row)
     conv = sum(1 for r in res_sat if r["converged"])
     return {
-        "requests": len(reqs), "arrival_rate_per_s": rate, "check_every": check_every,
+        "requests": len(reqs), "check_every": check_every,
         "max_active": max_active, "max_iterations": max_iterations, "stop_overflow": stop,
         "converged": conv,
         "placements_per_min": round(60 * len(reqs) / mk_sat, 1),
         "sequential_placements_per_min": round(60 * len(reqs) / seq_total, 1),
         "speedup_vs_sequential": round(seq_total / mk_sat, 3),
-        "p50_s": round(float(np.percentile(lat, 50)), 4), "p99_s": round(float(np.percentile(lat, 99)), 4),
-        "sequential_p50_s": round(float(np.percentile(seq_lat, 50)), 4),
-        "sequential_p99_s": round(float(np.percentile(seq_lat, 99)), 4),
-        "poisson_makespan_s": round(mk_poi, 3), "max_batch": st_sat["max_batch"],
+        "p50_s": latency[-1]["p50_s"], "p99_s": latency[-1]["p99_s"],
+        "arrival_rate_per_s": latency[-1]["arrival_rate_per_s"],
+        "latency": latency, "max_batch": st_sat["max_batch"],
         "results_identical_to_alone": bool(same),
     }
 
@@ -213,7 +223,9 @@ def main():
                          "reference kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE scaling")
     ap.add_argument("--serving-requests", type=int, default=512,
                     help="requests of the serving measurement (0 skips it)")
-    ap.add_argument("--serving-rate", type=float, default=20.0)
+    ap.add_argument("--serving-rate", default="20,60",
+                    help="Poisson arrival rates (requests/s), comma separated; the last is the "
+                         "headline p50/p99 (60/s exceeds one-at-a-time capacity)")
     ap.add_argument("--serving-check-every", type=int, default=10)
     ap.add_argument("--serving-max-active", type=int, default=64)
     ap.add_argument("--serving-max-iterations", type=int, default=2000)
@@ -295,7 +307,8 @@ def main():
 
     serving = None
     if args.serving_requests > 0:
-        serving = serving_bench(args.serving_requests, args.serving_rate, args.serving_check_every,
+        rates = [float(r) for r in str(args.serving_rate).split(",") if r]
+        serving = serving_bench(args.serving_requests, rates, args.serving_check_every,
                                 args.serving_max_active, args.serving_max_iterations)
         print(f"[serving] {serving}", file=sys.stderr, flush=True)
 

@@ -238,6 +238,30 @@ int erp::launch_batched_iteration(const JobDesc *d, const BatchDims &bd, int nj,
     return rc;
 }
 
+// erp_run without graphs (no capture cost; bit-identical results): the serving
+// loop's first chunk of each job.
+int erp::run_eager(erp_ctx *ctx, int64_t iterations, erp_step_info *info) {
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    int rc = ERP_OK;
+    while (iterations > 0 && ctx->iter_host == 0) {
+        rc = phase_hpwl(ctx, djob(ctx));
+        if (!rc) rc = phase_potential(ctx);
+        if (!rc) rc = phase_nesterov(ctx);
+        if (!rc) rc = phase_scheduler(ctx);
+        if (rc) return rc;
+        --iterations;
+    }
+    for (int64_t k = 0; k < iterations && !rc; ++k) {
+        rc = launch_fused_iteration(ctx);
+        ctx->iter_host += 1;
+    }
+    if (rc) return rc;
+    if (info) return read_info(ctx, info);
+    ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return ERP_OK;
+}
+
 size_t erp::estimate_ctx_bytes(const erp_design *d, const erp_config *cfg) {
     const size_t N = (size_t)d->num_nodes, C = (size_t)d->num_movable, P = (size_t)d->num_pins_total;
     const size_t E = (size_t)d->num_nets, B = (size_t)cfg->num_bins_x * (size_t)cfg->num_bins_y;

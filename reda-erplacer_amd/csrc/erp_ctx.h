@@ -44,6 +44,8 @@ namespace erp {
 // Context helpers defined in erp_api.hip.
 int upload_job(erp_ctx *ctx);
 int read_info(erp_ctx *ctx, erp_step_info *info);
+// `iterations` steps with eager launches (erp_run without graph capture).
+int run_eager(erp_ctx *ctx, int64_t iterations, erp_step_info *info);
 // Launch one fused iteration of a batch of jobs that share a bin grid.
 int launch_batched_iteration(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
 // Device memory a context for this design/config would hold.

@@ -147,6 +147,8 @@ int launch_combine(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s)
 int launch_cell_fused(const JobDesc *d, const BatchDims &bd, int nj, bool store_parts,
                       hipStream_t s);
 int launch_update(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
+// out[3j..3j+2] = hpwl, overflow, nonfinite of job j
+int launch_gather_state(const JobDesc *d, int nj, double *out, hipStream_t s);
 int launch_scalar(const JobDesc *d, int nj, int ops, hipStream_t s);
 int launch_potential_fft(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
                          KernelTimer *t = nullptr);

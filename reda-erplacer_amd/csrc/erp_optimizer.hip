@@ -340,6 +340,16 @@ __global__ void __launch_bounds__(kScalarThreads) k_scalar(const JobDesc *__rest
     s.nonfinite = isfinite(chk) ? 0 : 1;
 }
 
+// (hpwl, overflow, nonfinite) of every job of a batch into one array.
+__global__ void k_gather_state(const JobDesc *__restrict__ jd, int nj, double *__restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nj) return;
+    const DevState *st = jd[j].sa.st;
+    out[3 * j] = st->hpwl;
+    out[3 * j + 1] = st->overflow;
+    out[3 * j + 2] = (double)st->nonfinite;
+}
+
 __global__ void k_reset_state(DevState *st, double gamma0, double dw0) {
     DevState s = {};
     s.iteration = 0;
@@ -424,6 +434,13 @@ int launch_update(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s) 
 
 int launch_scalar(const JobDesc *d, int nj, int ops, hipStream_t s) {
     hipLaunchKernelGGL(k_scalar, dim3(nj), dim3(kScalarThreads), 0, s, d, ops);
+    ERP_HIP_TRY(hipGetLastError());
+    return ERP_OK;
+}
+
+int launch_gather_state(const JobDesc *d, int nj, double *out, hipStream_t s) {
+    if (nj <= 0) return ERP_OK;
+    hipLaunchKernelGGL(k_gather_state, dim3((unsigned)((nj + 63) / 64)), dim3(64), 0, s, d, nj, out);
     ERP_HIP_TRY(hipGetLastError());
     return ERP_OK;
 }

@@ -80,6 +80,16 @@ struct Group {
     DevBuf table;                   // JobDesc[members]
     BatchDims dims;
     hipGraphExec_t graph = nullptr;
+    DevBuf state_dev;               // (hpwl, overflow, nonfinite) per member, written by the graph
+    double *state_host = nullptr;   // pinned copy, read after each round
+    int capacity = 0;
+    Group() = default;
+    Group(const Group &) = delete;
+    Group &operator=(const Group &) = delete;
+    ~Group() {
+        drop_graph();
+        if (state_host) (void)hipHostFree(state_host);
+    }
     void drop_graph() {
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
@@ -164,12 +174,14 @@ void admit(erp_server *s) {
         erp_step_info info{};
         if (!rc) {
             erp_set_store_gradients(j->ctx, 0);
+            // the first chunk runs alone with eager launches (no per-job graph
+            // capture; results are bit-identical to the graph-replayed loop)
             const int64_t first = std::min<int64_t>(s->cfg.check_every, j->max_iterations);
-            rc = erp_run(j->ctx, first, &info);
+            rc = run_eager(j->ctx, first, &info);
             j->iterations = first;
             const int64_t left = j->max_iterations - j->iterations;
             if (!rc && !finished_now(j, info) && left < s->cfg.check_every) {
-                rc = erp_run(j->ctx, left, &info);
+                rc = run_eager(j->ctx, left, &info);
                 j->iterations += left;
             }
         }
@@ -214,12 +226,21 @@ int rebuild(erp_server *s, Group &g) {
     }
     int rc = g.table.alloc(sizeof(JobDesc) * tab.size());
     if (rc) return rc;
+    if (g.capacity < (int)tab.size()) {
+        const int cap = std::max<int>((int)tab.size(), s->cfg.max_active);
+        if ((rc = g.state_dev.alloc(sizeof(double) * 3 * (size_t)cap))) return rc;
+        if (g.state_host) (void)hipHostFree(g.state_host);
+        g.state_host = nullptr;
+        ERP_HIP_TRY(hipHostMalloc((void **)&g.state_host, sizeof(double) * 3 * (size_t)cap, hipHostMallocDefault));
+        g.capacity = cap;
+    }
     ERP_HIP_TRY(hipMemcpyAsync(g.table.p, tab.data(), sizeof(JobDesc) * tab.size(), hipMemcpyHostToDevice, s->stream));
     ERP_HIP_TRY(hipStreamSynchronize(s->stream));
     const int nj = (int)tab.size();
     ERP_HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < s->cfg.check_every && !rc; ++k)
         rc = launch_batched_iteration(g.table.as<JobDesc>(), g.dims, nj, s->stream);
+    if (!rc) rc = launch_gather_state(g.table.as<JobDesc>(), nj, g.state_dev.as<double>(), s->stream);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(s->stream, &graph);
     if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
@@ -236,6 +257,7 @@ int rebuild(erp_server *s, Group &g) {
 // job is checked.
 int round(erp_server *s) {
     std::vector<Job *> running;
+    std::vector<const double *> state;      // (hpwl, overflow, nonfinite) of running[k]
     for (auto &kv : s->groups) {
         Group &g = kv.second;
         if (g.dirty) {
@@ -248,12 +270,14 @@ int round(erp_server *s) {
             std::lock_guard<std::mutex> lk(s->mu);
             ++s->batched_launches;
         }
-        for (Job *j : g.members) running.push_back(j);
+        ERP_HIP_TRY(hipMemcpyAsync(g.state_host, g.state_dev.p, sizeof(double) * 3 * g.members.size(),
+                                   hipMemcpyDeviceToHost, s->stream));
+        for (size_t k = 0; k < g.members.size(); ++k) {
+            running.push_back(g.members[k]);
+            state.push_back(g.state_host + 3 * k);
+        }
     }
     if (running.empty()) return ERP_OK;
-    std::vector<DevState> st(running.size());
-    for (size_t k = 0; k < running.size(); ++k)
-        ERP_HIP_TRY(hipMemcpyAsync(&st[k], running[k]->ctx->state.p, sizeof(DevState), hipMemcpyDeviceToHost, s->stream));
     ERP_HIP_TRY(hipStreamSynchronize(s->stream));
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -264,13 +288,13 @@ int round(erp_server *s) {
         j->iterations += s->cfg.check_every;
         j->ctx->iter_host += s->cfg.check_every;
         erp_step_info info{};
-        info.hpwl = st[k].hpwl;
-        info.overflow = st[k].overflow;
-        info.nonfinite = st[k].nonfinite;
+        info.hpwl = state[k][0];
+        info.overflow = state[k][1];
+        info.nonfinite = (int32_t)state[k][2];
         const int64_t left = j->max_iterations - j->iterations;
         if (!finished_now(j, info) && left < s->cfg.check_every) {
             // the last, shorter chunk runs alone (as erp_run_until would)
-            int rc = erp_run(j->ctx, left, &info);
+            int rc = run_eager(j->ctx, left, &info);
             if (rc) return rc;
             j->iterations += left;
         }
