@@ -184,6 +184,17 @@ __device__ __forceinline__ int sort_key(float lx, float ly, float inv_bw, float 
     return (int)((spread_bits((unsigned)(bx >> shift)) << 1) | spread_bits((unsigned)(by >> shift)));
 }
 
+__device__ __forceinline__ unsigned compact_bits(unsigned v) {  // 0b a0b0c -> 0b abc
+    v &= 0x5555u;
+    v = (v | (v >> 1)) & 0x3333u;
+    v = (v | (v >> 2)) & 0x0f0fu;
+    v = (v | (v >> 4)) & 0x00ffu;
+    return v;
+}
+__device__ __forceinline__ int tile_key(int tx, int ty) {
+    return (int)((spread_bits((unsigned)tx) << 1) | spread_bits((unsigned)ty));
+}
+
 __global__ void __launch_bounds__(kBlock) k_sort_keys(const JobDesc *__restrict__ jd, int shift,
                                                       int ntiles) {
     __shared__ int lh[kSortTiles];
@@ -238,19 +249,11 @@ __global__ void __launch_bounds__(kBlock) k_sort_keys(const JobDesc *__restrict_
         if (lh[t]) atomicAdd(&hist[t], lh[t]);
 }
 
-__global__ void __launch_bounds__(1024) k_sort_scan(const JobDesc *__restrict__ jd, int ntiles) {
-    __shared__ int wsum[16];
-    int32_t *__restrict__ hist = jd[blockIdx.x].sb.hist;
-    int32_t *__restrict__ cursor = jd[blockIdx.x].sb.cursor;
-    const int per = (ntiles + 1023) / 1024;          // ntiles <= kSortTiles = 4 * 1024
-    const int t0 = threadIdx.x * per;
-    int v[4] = {0, 0, 0, 0};
-    int sum = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (k < per && t0 + k < ntiles) { v[k] = hist[t0 + k]; sum += v[k]; }
-    // inclusive scan of the per-thread sums: within the wavefront by shuffles,
-    // then across the 16 wavefronts
+// Exclusive scan of the tile counts into the placement cursors and the tile
+// starts; then, for the owner scatter, the number of workgroups each tile needs
+// (its own cells and those of the `reach` tiles to its left and below, in chunks
+// of kOwnerVisits; 0 outside the grid) and their exclusive scan.
+__device__ __forceinline__ int block_exclusive_scan(int sum, int *wsum, int &total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int inc = sum;
 #pragma unroll
@@ -261,16 +264,68 @@ __global__ void __launch_bounds__(1024) k_sort_scan(const JobDesc *__restrict__ 
     if (lane == 63) wsum[wave] = inc;
     __syncthreads();
     int before = 0;
-    for (int w = 0; w < wave; ++w) before += wsum[w];
-    int run = before + inc - sum;                    // exclusive prefix of this thread
+    total = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        if (w < wave) before += wsum[w];
+        total += wsum[w];
+    }
+    __syncthreads();
+    return before + inc - sum;
+}
+
+__global__ void __launch_bounds__(1024) k_sort_scan(const JobDesc *__restrict__ jd, int ntiles, int shift,
+                                                   int owner_items) {
+    __shared__ int wsum[16];
+    __shared__ int cnt[kSortTiles];
+    const SortBufs &sb = jd[blockIdx.x].sb;
+    int32_t *__restrict__ hist = sb.hist;
+    int32_t *__restrict__ cursor = sb.cursor;
+    const int per = (ntiles + 1023) / 1024;          // ntiles <= kSortTiles = 4 * 1024
+    const int t0 = threadIdx.x * per;
+    int v[4] = {0, 0, 0, 0};
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (k < per && t0 + k < ntiles) { v[k] = hist[t0 + k]; sum += v[k]; cnt[t0 + k] = v[k]; }
+    int total;
+    int run = block_exclusive_scan(sum, wsum, total);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (k < per && t0 + k < ntiles) {
             cursor[t0 + k] = run;
+            sb.tstart[t0 + k] = run;
             run += v[k];
             hist[t0 + k] = 0;
         }
     }
+    if (threadIdx.x == 0) sb.tstart[ntiles] = total;
+    if (!owner_items) return;                        // uniform
+    // owner-scatter work items (cnt is complete: the scan above synchronised)
+    const int T = 1 << shift, R = sb.reach;
+    const int tiles_x = (sb.m + T - 1) >> shift, tiles_y = (sb.n + T - 1) >> shift;
+    int nch[4] = {0, 0, 0, 0};
+    sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int t = t0 + k;
+        if (k >= per || t >= ntiles) continue;
+        const int tx = (int)compact_bits((unsigned)t >> 1), ty = (int)compact_bits((unsigned)t);
+        if (tx >= tiles_x || ty >= tiles_y) continue;
+        int vis = 0;
+        for (int a = 0; a <= R && a <= tx; ++a)
+            for (int b = 0; b <= R && b <= ty; ++b) vis += cnt[tile_key(tx - a, ty - b)];
+        nch[k] = vis ? (vis + kOwnerVisits - 1) / kOwnerVisits : 1;
+        sum += nch[k];
+    }
+    run = block_exclusive_scan(sum, wsum, total);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k < per && t0 + k < ntiles) {
+            sb.items[t0 + k] = run;
+            run += nch[k];
+        }
+    }
+    if (threadIdx.x == 0) sb.items[ntiles] = total;
 }
 
 __global__ void __launch_bounds__(kBlock) k_sort_place(const JobDesc *__restrict__ jd, int ntiles) {
@@ -337,6 +392,95 @@ __global__ void __launch_bounds__(kBlock) k_sort_place(const JobDesc *__restrict
             rratio[dst] = crat[j];
             roff[dst] = make_float2(cst[j].z, cst[j].w);
         }
+    }
+}
+
+// Owner scatter: workgroup b takes item b of the list built by k_sort_scan --
+// a bin tile (T x T bins, Morton index t) and a chunk of the cells that can touch
+// it: the sorted cells of tile t and of the `reach` tiles to its left and below
+// (a cell's lower-left bin lies in its own tile and it spans at most `reach`
+// tiles further).  Contributions are clipped to the tile and summed in an LDS
+// int64 tile; a tile served by one workgroup stores its bins (every bin, no
+// atomics, the map needs no clearing there); a tile split over several
+// workgroups adds its nonzero bins atomically into the cleared map.  The
+// workgroup that visits a cell in its own tile also copies the cell's sorted
+// record back as the next sort's input.
+__global__ void __launch_bounds__(kBlock) k_scatter_owner(const JobDesc *__restrict__ jd, int shift,
+                                                          int ntiles) {
+    __shared__ unsigned long long tile[32 * 32];
+    __shared__ int src_start[9], src_cnt[9];
+    const SortBufs &sb = jd[blockIdx.y].sb;
+    const int32_t *__restrict__ items = sb.items;
+    const int b = (int)blockIdx.x;
+    if (b >= items[ntiles]) return;                 // uniform per workgroup
+    int lo = 0, hi = ntiles;                        // last t with items[t] <= b
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (items[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    const int t = lo;
+    const int chunk = b - items[t];
+    const bool exclusive = items[t + 1] - items[t] == 1;
+    const int T = 1 << shift, R = sb.reach, W = R + 1;
+    const int tx = (int)compact_bits((unsigned)t >> 1), ty = (int)compact_bits((unsigned)t);
+    const int x0 = tx << shift, y0 = ty << shift;
+    const int m = sb.m, n = sb.n;
+    if ((int)threadIdx.x < W * W) {                 // own tile first
+        const int a = threadIdx.x % W, c = threadIdx.x / W;
+        int st = 0, ct = 0;
+        if (a <= tx && c <= ty) {
+            const int s = tile_key(tx - a, ty - c);
+            st = sb.tstart[s];
+            ct = sb.tstart[s + 1] - st;
+        }
+        src_start[threadIdx.x] = st;
+        src_cnt[threadIdx.x] = ct;
+    }
+    for (int e = threadIdx.x; e < T * T; e += kBlock) tile[e] = 0ull;
+    __syncthreads();
+    int total = 0;
+    for (int k = 0; k < W * W; ++k) total += src_cnt[k];
+    const int v0 = chunk * kOwnerVisits, v1 = min(v0 + kOwnerVisits, total);
+    const float4 *__restrict__ rec = sb.rec;
+    const float *__restrict__ rratio = sb.rratio;
+    const float bw = sb.bw, bh = sb.bh;
+    const float inv_bw = 1.f / bw, inv_bh = 1.f / bh;
+    const double scale = jd[blockIdx.y].pb.scale;
+    for (int v = v0 + (int)threadIdx.x; v < v1; v += kBlock) {
+        int k = 0, off = v;
+        while (off >= src_cnt[k]) { off -= src_cnt[k]; ++k; }
+        const int i = src_start[k] + off;
+        const float4 r4 = rec[i];
+        const float r = rratio[i];
+        if (k == 0) {                               // the cell's own tile
+            sb.perm[i] = sb.perm_out[i];
+            const float2 o = sb.roff[i];
+            sb.sstat[i] = make_float4(r4.z, r4.w, o.x, o.y);
+            sb.sratio[i] = r;
+        }
+        int bxl, bxh, byl, byh;
+        bin_range(r4.x, r4.z, inv_bw, m, bxl, bxh);
+        bin_range(r4.y, r4.w, inv_bh, n, byl, byh);
+        bxl = max(bxl, x0); bxh = min(bxh, x0 + T);
+        byl = max(byl, y0); byh = min(byh, y0 + T);
+        for (int bx = bxl; bx < bxh; ++bx) {
+            const float px = r * tri_overlap(r4.x, r4.z, bx, bw);
+            for (int by = byl; by < byh; ++by) {
+                const float py = tri_overlap(r4.y, r4.w, by, bh);
+                const int64_t q = quantize(px * py, scale);
+                if (q != 0) atomicAdd(&tile[(bx - x0) * T + (by - y0)], (unsigned long long)q);
+            }
+        }
+    }
+    __syncthreads();
+    int64_t *__restrict__ qm = jd[blockIdx.y].pb.q_mov;
+    for (int e = threadIdx.x; e < T * T; e += kBlock) {
+        const int bx = x0 + e / T, by = y0 + e % T;
+        if (bx >= m || by >= n) continue;
+        const unsigned long long val = tile[e];
+        if (exclusive) qm[(size_t)bx * n + by] = (int64_t)val;
+        else if (val != 0ull) add_q(qm + (size_t)bx * n + by, (int64_t)val);
     }
 }
 
@@ -424,6 +568,17 @@ int launch_scatter(const JobDesc *d, const BatchDims &bd, int nj, bool copy_perm
     return ERP_OK;
 }
 
+int launch_scatter_owner(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s) {
+    if (bd.max_cells <= 0) return ERP_OK;
+    const SortGeom g = sort_geometry(bd.m, bd.n);
+    const int64_t w = bd.max_reach + 1;
+    // upper bound of the items of any job: one per tile plus the chunks
+    const int64_t nb = g.ntiles + (w * w * bd.max_cells + kOwnerVisits - 1) / kOwnerVisits;
+    hipLaunchKernelGGL(k_scatter_owner, dim3((unsigned)nb, nj), dim3(kBlock), 0, s, d, g.shift, g.ntiles);
+    ERP_HIP_TRY(hipGetLastError());
+    return ERP_OK;
+}
+
 SortGeom sort_geometry(int m, int n) {
     SortGeom g;
     // Morton keys over a square of side^2 tiles, side = next power of two of the
@@ -438,7 +593,8 @@ SortGeom sort_geometry(int m, int n) {
     return g;
 }
 
-int launch_sort(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s, KernelTimer *t) {
+int launch_sort(const JobDesc *d, const BatchDims &bd, int nj, bool owner_items, hipStream_t s,
+                KernelTimer *t) {
     if (bd.max_cells <= 0) return ERP_OK;
     const SortGeom g = sort_geometry(bd.m, bd.n);
     const unsigned nb = (unsigned)((bd.max_cells + kScatterCells - 1) / kScatterCells);
@@ -447,7 +603,7 @@ int launch_sort(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s, Ke
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
     if (t) t->begin("sort_scan");
-    hipLaunchKernelGGL(k_sort_scan, dim3(nj), dim3(1024), 0, s, d, g.ntiles);
+    hipLaunchKernelGGL(k_sort_scan, dim3(nj), dim3(1024), 0, s, d, g.ntiles, g.shift, owner_items ? 1 : 0);
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
     if (t) t->begin("sort_place");

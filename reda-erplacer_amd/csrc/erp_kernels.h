@@ -11,6 +11,7 @@ constexpr int kScatterCells = kScatterPer * kBlock;   // cells per workgroup
 constexpr int kScatterTile = 4096;                    // LDS int64 bins per workgroup
 constexpr int kScatterSide = 64;                      // tile side when the box is too big
 constexpr int kSortTiles = 4096;                      // max sort tiles (64 x 64 Morton)
+constexpr int kOwnerVisits = 4096;                    // cell visits per owner-scatter workgroup
 struct SortGeom {
     int shift, ntiles;
 };
@@ -32,6 +33,10 @@ struct SortBufs {
     float *sratio;
     float2 *ll;          // [C] clamped lower-left corners in the order of perm (sort temp)
     float2 *roff;        // [C] clamp offsets in sorted order
+    // owner scatter: first sorted index of every tile ([ntiles + 1], Morton
+    // order) and the workgroup items per tile (exclusive prefix, [ntiles + 1])
+    int32_t *tstart, *items;
+    int reach;           // tiles a cell can reach beyond its own, per axis (>= 1)
 };
 int launch_init_sort_static(const float2 *csize, const float2 *coff, const float *ratio,
                             int64_t num_cells, float4 *sstat, float *sratio, hipStream_t s);
@@ -131,15 +136,20 @@ struct BatchDims {
     int64_t max_cells = 0;              // movable cells
     int64_t max_n2 = 0;                 // two-pin nets
     int64_t max_bucket[kWaBuckets] = {};
+    int max_reach = 1;                  // owner scatter neighbourhood
     void add(const JobDesc &j);         // widen to include job j
 };
 
 int launch_wa(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
               KernelTimer *t = nullptr);
-int launch_sort(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
-                KernelTimer *t = nullptr);
+int launch_sort(const JobDesc *d, const BatchDims &bd, int nj, bool owner_items, hipStream_t s,
+                KernelTimer *t = nullptr);   // owner_items: also build the owner-scatter work list
 // perm_out -> perm is copied by the scatter when copy_perm is set
 int launch_scatter(const JobDesc *d, const BatchDims &bd, int nj, bool copy_perm, hipStream_t s);
+// Density scatter of the sorted cells, one workgroup per bin tile (plain stores
+// of the tile's bins; atomics only for tiles split over several workgroups), and
+// the sorted order copied back as the next sort's input.
+int launch_scatter_owner(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
 int launch_pin2cell(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
 int launch_efield(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
 int launch_l1(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);

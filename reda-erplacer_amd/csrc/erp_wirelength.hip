@@ -310,6 +310,7 @@ void BatchDims::add(const JobDesc &j) {
     max_n2 = std::max<int64_t>(max_n2, j.wa.n2);
     for (int b = 0; b < kWaBuckets; ++b)
         max_bucket[b] = std::max<int64_t>(max_bucket[b], j.wa.bucket_begin[b + 1] - j.wa.bucket_begin[b]);
+    max_reach = std::max(max_reach, j.sb.reach);
 }
 
 int wa_partials(WaNets &w) {
