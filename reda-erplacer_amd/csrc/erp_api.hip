@@ -151,11 +151,23 @@ int phase_scheduler(erp_ctx *ctx) {
 }
 
 // One iteration (iteration >= 1) as a fused kernel sequence: what the graphs hold.
-int launch_fused_iteration(erp_ctx *ctx) {
+// fork: (graph capture only) the WA kernel runs on a second branch beside the
+// sort / scatter / spectral chain -- both only read the positions -- and joins
+// before the cell kernel.
+int launch_fused_iteration(erp_ctx *ctx, bool fork = false) {
     KernelTimer *t = ctx->timing ? ctx->timer : nullptr;
-    int rc = launch_wa(djob(ctx), ctx->dims, 1, ctx->stream, t);
-    if (rc) return rc;
-    if ((rc = run_potential(ctx))) return rc;
+    int rc;
+    if (fork) {
+        ERP_HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
+        ERP_HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+        if ((rc = launch_wa(djob(ctx), ctx->dims, 1, ctx->side))) return rc;
+        ERP_HIP_TRY(hipEventRecord(ctx->ev_join, ctx->side));
+        if ((rc = run_potential(ctx))) return rc;
+        ERP_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+    } else {
+        if ((rc = launch_wa(djob(ctx), ctx->dims, 1, ctx->stream, t))) return rc;
+        if ((rc = run_potential(ctx))) return rc;
+    }
     if (t) t->begin("cell_fused");
     rc = launch_cell_fused(djob(ctx), ctx->dims, 1, ctx->debug_parts, ctx->stream);
     if (t) t->end();
@@ -179,7 +191,7 @@ void drop_graphs(erp_ctx *ctx) {
 int capture(erp_ctx *ctx, int iters, hipGraphExec_t *out) {
     ERP_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     int rc = ERP_OK;
-    for (int k = 0; k < iters && rc == ERP_OK; ++k) rc = launch_fused_iteration(ctx);
+    for (int k = 0; k < iters && rc == ERP_OK; ++k) rc = launch_fused_iteration(ctx, true);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -420,7 +432,13 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     ctx->device = cfg->device;
     DeviceGuard g(ctx->device);
     if (g.rc) { delete ctx; return g.rc; }
-    if (hipStreamCreate(&ctx->stream) != hipSuccess) {
+    if (hipStreamCreate(&ctx->stream) != hipSuccess || hipStreamCreate(&ctx->side) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
+        if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+        if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+        if (ctx->side) (void)hipStreamDestroy(ctx->side);
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return fail(ERP_ERR_HIP, "hipStreamCreate failed");
     }
@@ -668,8 +686,12 @@ int erp_ctx_destroy(erp_ctx *ctx) {
     drop_graphs(ctx);
     if (ctx->mfma) mfma_plan_destroy(ctx->mfma);
     delete ctx->timer;
-    hipStream_t s = ctx->stream;
+    hipStream_t s = ctx->stream, side = ctx->side;
+    hipEvent_t e1 = ctx->ev_fork, e2 = ctx->ev_join;
     delete ctx;   // releases every DevBuf
+    if (e1) (void)hipEventDestroy(e1);
+    if (e2) (void)hipEventDestroy(e2);
+    if (side) (void)hipStreamDestroy(side);
     if (s) (void)hipStreamDestroy(s);
     return ERP_OK;
 }
