@@ -79,6 +79,41 @@ class Design:
         }
 
 
+def from_nets(x, y, w, h, num_movable, nets, die_w, die_h, name="custom") -> Design:
+    """A Design from instance arrays (movable first) and nets given as lists of
+    (instance, pin offset x, pin offset y).  Two-pin nets are placed first (the
+    contract's n_2nets prefix); pins are numbered net by net."""
+    x, y, w, h = (np.asarray(a, np.float32) for a in (x, y, w, h))
+    N = int(x.shape[0])
+    order = sorted(range(len(nets)), key=lambda e: 0 if len(nets[e]) == 2 else 1)
+    n2 = sum(1 for e in nets if len(e) == 2)
+    pin_node, pox, poy, net_start = [], [], [], [0]
+    for e in order:
+        for node, ox, oy in nets[e]:
+            pin_node.append(int(node))
+            pox.append(float(ox))
+            poy.append(float(oy))
+        net_start.append(len(pin_node))
+    pin_node = np.asarray(pin_node, np.int64)
+    counts = np.bincount(pin_node, minlength=N) if pin_node.size else np.zeros(N, np.int64)
+    node_pin_start = np.zeros(N + 1, np.int64)
+    np.cumsum(counts, out=node_pin_start[1:])
+    node_pins = np.argsort(pin_node, kind="stable")
+    movable_area = float((w[:num_movable].astype(np.float64) * h[:num_movable]).sum())
+    fixed_area = float((w[num_movable:].astype(np.float64) * h[num_movable:]).sum())
+    return Design(
+        name=name, seed=0, die_w=float(die_w), die_h=float(die_h), num_movable=int(num_movable),
+        x=x, y=y, w=w, h=h, num_pins=counts.astype(np.float32),
+        pin_offset_x=np.asarray(pox, np.float32), pin_offset_y=np.asarray(poy, np.float32),
+        pin_node=pin_node.astype(np.int32), net_start=np.asarray(net_start, np.int32),
+        net_pins=np.arange(pin_node.size, dtype=np.int32), n_2nets=n2,
+        node_pin_start=node_pin_start.astype(np.int32), node_pins=node_pins.astype(np.int32),
+        movable_area=movable_area,
+        cell_utilization=movable_area / max(float(die_w) * float(die_h) - fixed_area, 1.0),
+        params={},
+    )
+
+
 def _tail_degrees(rng, count, lo, cap, alpha):
     """Power-law degrees d in [lo, cap] with P(d) proportional to d**-alpha."""
     if count == 0:
