@@ -200,6 +200,9 @@ typedef struct {
     int64_t pending, active, finished, bytes_in_use;
     int64_t rounds, batched_launches, max_batch;
     double clock;               /* seconds since erp_server_create */
+    /* worker time split: admitting (context creation + first chunk), batch
+     * rounds (graph launches, capture, state reads), retiring (copy-out, free) */
+    double admit_seconds, round_seconds, retire_seconds;
 } erp_server_stat;
 
 void erp_server_config_default(erp_server_config *config);

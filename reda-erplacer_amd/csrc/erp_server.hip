@@ -1,12 +1,14 @@
 // Serving form: a persistent request loop that runs many placements at once on
 // one GPU with continuous batching.
 //
-// Requests (designs) are submitted from any thread and queue up.  A worker
-// thread owns the GPU: it admits queued jobs in arrival order while the job
-// slots and the HBM budget allow (a job that does not fit waits, later smaller
-// ones may pass it), creates their contexts (state stays resident in device
-// memory), runs each new job's first `check_every` iterations on its own, and
-// then adds it to the batch of its bin grid.  Each batch is one hipGraph that
+// Requests (designs) are submitted from any thread and queue up.  An admission
+// thread takes queued jobs in arrival order while the job slots and the HBM
+// budget allow (a job that does not fit waits, later smaller ones may pass it),
+// creates their contexts (state stays resident in device memory) and runs each
+// new job's first `check_every` iterations on the job's own stream, while the
+// batch worker keeps the running batches going on the server's stream; the
+// worker then adds the prepared job to the batch of its bin grid.  Each batch is
+// one hipGraph that
 // holds `check_every` iterations of every member, every kernel launched once for
 // all members (job = blockIdx.y).  After each round the worker reads every
 // job's scalars; a job whose overflow fell below its stop threshold, that
@@ -104,19 +106,24 @@ struct erp_server {
     Clock::time_point t0;
     std::mutex mu;
     std::condition_variable cv, cv_done;
+    std::condition_variable cv_admit;                   // wakes the admission thread
     std::deque<std::unique_ptr<Job>> pending;
-    std::map<int64_t, std::unique_ptr<Job>> active;     // admitted, not finished
+    std::map<int64_t, std::unique_ptr<Job>> preparing;  // context being created / first chunk
+    std::vector<std::unique_ptr<Job>> ready;            // prepared, not yet in a batch
+    std::map<int64_t, std::unique_ptr<Job>> active;     // in a batch, not finished
     std::map<int64_t, std::unique_ptr<Job>> finished;   // results not yet released
     std::vector<int64_t> unpolled;
     std::map<std::pair<int, int>, Group> groups;
     int64_t next_id = 1;
-    int64_t admitting = 0;        // job between the queue and the active set
     size_t bytes_in_use = 0;
     int64_t rounds = 0, batched_launches = 0, max_batch = 0;
+    double admit_s = 0, round_s = 0, retire_s = 0;
     bool stop = false;
     int worker_rc = ERP_OK;
     std::string worker_error;
-    std::thread worker;
+    std::thread worker, admitter;
+    // jobs holding a slot (admitted and not finished)
+    size_t resident() const { return preparing.size() + ready.size() + active.size(); }
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
 
@@ -128,8 +135,10 @@ void fail_job(erp_server *s, Job *j, int rc) {
     j->done = true;
 }
 
-// Retire a finished job: copy results out and free its device memory.
-void retire(erp_server *s, Job *j, const erp_step_info &info) {
+// Retire a finished job: copy results out and free its device memory.  Returns
+// the seconds it took (the caller adds them to the stats under the lock).
+double retire(erp_server *s, Job *j, const erp_step_info &info) {
+    const double t0 = s->now();
     erp_ctx *c = j->ctx;
     j->result.iterations = j->iterations;
     j->result.hpwl = info.hpwl;
@@ -146,31 +155,42 @@ void retire(erp_server *s, Job *j, const erp_step_info &info) {
     erp_ctx_destroy(c);
     j->ctx = nullptr;
     j->done = true;
+    return s->now() - t0;
 }
 
 bool finished_now(const Job *j, const erp_step_info &info) {
     return info.nonfinite || info.overflow < j->stop_overflow || j->iterations >= j->max_iterations;
 }
 
-// Admit queued jobs that fit; each runs its first chunk alone.
-void admit(erp_server *s) {
+// Admission thread: take a queued job that fits, prepare it outside the lock
+// (context + first chunk on the job's own stream), hand it to the worker.
+void admitter_main(erp_server *s) {
+    const bool dev_ok = hipSetDevice(s->cfg.device) == hipSuccess;
     for (;;) {
         std::unique_ptr<Job> job;
+        Job *j = nullptr;
         {
-            std::lock_guard<std::mutex> lk(s->mu);
-            if ((int)s->active.size() >= s->cfg.max_active) return;
-            auto it = std::find_if(s->pending.begin(), s->pending.end(), [&](const std::unique_ptr<Job> &j) {
-                return s->bytes_in_use + j->bytes <= (size_t)s->cfg.hbm_budget_bytes;
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_admit.wait(lk, [&] {
+                if (s->stop || s->worker_rc != ERP_OK) return true;
+                if ((int)s->resident() >= s->cfg.max_active) return false;
+                return std::any_of(s->pending.begin(), s->pending.end(), [&](const std::unique_ptr<Job> &q) {
+                    return s->bytes_in_use + q->bytes <= (size_t)s->cfg.hbm_budget_bytes;
+                });
             });
-            if (it == s->pending.end()) return;
-            job = std::move(*it);
+            if (s->stop || s->worker_rc != ERP_OK) return;
+            auto it = std::find_if(s->pending.begin(), s->pending.end(), [&](const std::unique_ptr<Job> &q) {
+                return s->bytes_in_use + q->bytes <= (size_t)s->cfg.hbm_budget_bytes;
+            });
+            j = it->get();
+            s->bytes_in_use += j->bytes;
+            s->preparing[j->id] = std::move(*it);
             s->pending.erase(it);
-            s->bytes_in_use += job->bytes;
-            s->admitting = job->id;
         }
-        Job *j = job.get();
-        j->result.t_admit = s->now();
-        int rc = erp_ctx_create(&j->design.d, &j->cfg, &j->ctx);
+        const double ta = s->now();
+        j->result.t_admit = ta;
+        int rc = dev_ok ? erp_ctx_create(&j->design.d, &j->cfg, &j->ctx)
+                        : fail(ERP_ERR_HIP, "hipSetDevice failed in the admission thread");
         erp_step_info info{};
         if (!rc) {
             erp_set_store_gradients(j->ctx, 0);
@@ -185,32 +205,34 @@ void admit(erp_server *s) {
                 j->iterations += left;
             }
         }
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->admitting = 0;
+        double dt_retire = 0.0;
         if (rc) {
             if (j->ctx) erp_ctx_destroy(j->ctx);
             j->ctx = nullptr;
-            s->bytes_in_use -= j->bytes;
             fail_job(s, j, rc);
-            s->unpolled.push_back(j->id);
-            s->finished[j->id] = std::move(job);
-            s->cv_done.notify_all();
-            continue;
+        } else if (finished_now(j, info)) {
+            dt_retire = retire(s, j, info);
         }
-        if (finished_now(j, info)) {
-            retire(s, j, info);
+        std::lock_guard<std::mutex> lk(s->mu);
+        job = std::move(s->preparing[j->id]);
+        s->preparing.erase(j->id);
+        s->admit_s += s->now() - ta;
+        s->retire_s += dt_retire;
+        if (!j->done && s->worker_rc != ERP_OK) {    // the worker stopped meanwhile
+            erp_ctx_destroy(j->ctx);
+            j->ctx = nullptr;
+            fail_job(s, j, s->worker_rc);
+        }
+        if (j->done) {
             s->bytes_in_use -= j->bytes;
             s->unpolled.push_back(j->id);
             s->finished[j->id] = std::move(job);
             s->cv_done.notify_all();
-            continue;
+            s->cv_admit.notify_all();
+        } else {
+            s->ready.push_back(std::move(job));
+            s->cv.notify_all();
         }
-        Group &g = s->groups[{j->ctx->m, j->ctx->n}];
-        g.m = j->ctx->m;
-        g.n = j->ctx->n;
-        g.members.push_back(j);
-        g.dirty = true;
-        s->active[j->id] = std::move(job);
     }
 }
 
@@ -299,8 +321,9 @@ int round(erp_server *s) {
             j->iterations += left;
         }
         if (!finished_now(j, info)) continue;
-        retire(s, j, info);
+        const double dt = retire(s, j, info);
         std::lock_guard<std::mutex> lk(s->mu);
+        s->retire_s += dt;
         for (auto &kv : s->groups) {
             auto &mem = kv.second.members;
             auto it = std::find(mem.begin(), mem.end(), j);
@@ -314,8 +337,33 @@ int round(erp_server *s) {
         s->finished[j->id] = std::move(s->active[j->id]);
         s->active.erase(j->id);
         s->cv_done.notify_all();
+        s->cv_admit.notify_all();           // a slot and memory are free
     }
     return ERP_OK;
+}
+
+// Fail every job that has not finished (after a HIP error in the worker).
+void fail_all(erp_server *s, int rc) {
+    auto finish = [&](std::unique_ptr<Job> &j) {
+        if (j->ctx) erp_ctx_destroy(j->ctx);
+        j->ctx = nullptr;
+        fail_job(s, j.get(), rc);
+        s->bytes_in_use -= j->bytes;
+        s->unpolled.push_back(j->id);
+        const int64_t id = j->id;
+        s->finished[id] = std::move(j);
+    };
+    for (auto &kv : s->active) finish(kv.second);
+    s->active.clear();
+    for (auto &j : s->ready) finish(j);
+    s->ready.clear();
+    for (auto &j : s->pending) {
+        s->bytes_in_use += j->bytes;       // finish() subtracts it
+        finish(j);
+    }
+    s->pending.clear();
+    for (auto &kv : s->groups) kv.second.drop_graph();
+    s->groups.clear();
 }
 
 void worker_main(erp_server *s) {
@@ -324,41 +372,37 @@ void worker_main(erp_server *s) {
         s->worker_rc = ERP_ERR_HIP;
         s->worker_error = "hipSetDevice failed in the serving worker";
         s->cv_done.notify_all();
+        s->cv_admit.notify_all();
         return;
     }
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv.wait(lk, [&] { return s->stop || !s->pending.empty() || !s->active.empty(); });
+            s->cv.wait(lk, [&] { return s->stop || !s->ready.empty() || !s->active.empty(); });
             if (s->stop) break;
+            for (auto &job : s->ready) {              // prepared jobs join their batches
+                Job *j = job.get();
+                Group &g = s->groups[{j->ctx->m, j->ctx->n}];
+                g.m = j->ctx->m;
+                g.n = j->ctx->n;
+                g.members.push_back(j);
+                g.dirty = true;
+                s->active[j->id] = std::move(job);
+            }
+            s->ready.clear();
         }
-        admit(s);
+        const double tb = s->now();
         int rc = round(s);
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->round_s += s->now() - tb;
         if (rc) {
-            std::lock_guard<std::mutex> lk(s->mu);
             s->worker_rc = rc;
             s->worker_error = erp_last_error();
-            // fail everything still queued or running; the server stops serving
-            for (auto &kv : s->active) {
-                if (kv.second->ctx) erp_ctx_destroy(kv.second->ctx);
-                kv.second->ctx = nullptr;
-                fail_job(s, kv.second.get(), rc);
-                s->unpolled.push_back(kv.first);
-                s->finished[kv.first] = std::move(kv.second);
-            }
-            s->active.clear();
-            for (auto &j : s->pending) {
-                fail_job(s, j.get(), rc);
-                s->unpolled.push_back(j->id);
-                s->finished[j->id] = std::move(j);
-            }
-            s->pending.clear();
-            for (auto &kv : s->groups) kv.second.drop_graph();
-            s->groups.clear();
+            fail_all(s, rc);                          // the server stops serving
             s->cv_done.notify_all();
+            s->cv_admit.notify_all();
             break;
         }
-        std::lock_guard<std::mutex> lk(s->mu);
         for (auto it = s->groups.begin(); it != s->groups.end();) {
             if (it->second.members.empty()) {
                 it->second.drop_graph();
@@ -399,6 +443,7 @@ int erp_server_create(const erp_server_config *cfg, erp_server **out) {
         return fail(ERP_ERR_HIP, "hipStreamCreate failed");
     }
     s->worker = std::thread(worker_main, s);
+    s->admitter = std::thread(admitter_main, s);
     *out = s;
     return ERP_OK;
 }
@@ -425,7 +470,7 @@ int erp_server_submit(erp_server *s, const erp_design *d, const erp_config *cfg,
     job->result.t_submit = s->now();
     *job_id = job->id;
     s->pending.push_back(std::move(job));
-    s->cv.notify_all();
+    s->cv_admit.notify_all();
     return ERP_OK;
 }
 
@@ -445,7 +490,9 @@ int erp_server_poll(erp_server *s, erp_job_result *out, int32_t capacity, int32_
 int erp_server_wait(erp_server *s, int64_t job_id, erp_job_result *out) {
     if (!s) return fail(ERP_ERR_INVALID, "null server");
     std::unique_lock<std::mutex> lk(s->mu);
-    const bool known = s->finished.count(job_id) || s->active.count(job_id) || s->admitting == job_id ||
+    const bool known = s->finished.count(job_id) || s->active.count(job_id) || s->preparing.count(job_id) ||
+                       std::any_of(s->ready.begin(), s->ready.end(),
+                                   [&](const std::unique_ptr<Job> &j) { return j->id == job_id; }) ||
                        std::any_of(s->pending.begin(), s->pending.end(),
                                    [&](const std::unique_ptr<Job> &j) { return j->id == job_id; });
     if (!known) return fail(ERP_ERR_INVALID, "unknown or released job id");
@@ -463,7 +510,7 @@ int erp_server_drain(erp_server *s) {
     if (!s) return fail(ERP_ERR_INVALID, "null server");
     std::unique_lock<std::mutex> lk(s->mu);
     s->cv_done.wait(lk, [&] {
-        return (s->pending.empty() && s->active.empty() && s->admitting == 0) || s->worker_rc != ERP_OK;
+        return (s->pending.empty() && s->resident() == 0) || s->worker_rc != ERP_OK;
     });
     if (s->worker_rc) return fail(s->worker_rc, s->worker_error);
     return ERP_OK;
@@ -491,13 +538,16 @@ int erp_server_stats(erp_server *s, erp_server_stat *st) {
     if (!s || !st) return fail(ERP_ERR_INVALID, "null argument");
     std::lock_guard<std::mutex> lk(s->mu);
     st->pending = (int64_t)s->pending.size();
-    st->active = (int64_t)s->active.size();
+    st->active = (int64_t)s->resident();
     st->finished = (int64_t)s->finished.size();
     st->bytes_in_use = (int64_t)s->bytes_in_use;
     st->rounds = s->rounds;
     st->batched_launches = s->batched_launches;
     st->max_batch = s->max_batch;
     st->clock = s->now();
+    st->admit_seconds = s->admit_s;
+    st->round_seconds = s->round_s;
+    st->retire_seconds = s->retire_s;
     return ERP_OK;
 }
 
@@ -508,11 +558,15 @@ int erp_server_destroy(erp_server *s) {
         s->stop = true;
         s->pending.clear();          // queued jobs are dropped
         s->cv.notify_all();
+        s->cv_admit.notify_all();
     }
+    if (s->admitter.joinable()) s->admitter.join();   // finishes the job it is preparing
     if (s->worker.joinable()) s->worker.join();
     (void)hipSetDevice(s->cfg.device);
     for (auto &kv : s->active)
         if (kv.second->ctx) erp_ctx_destroy(kv.second->ctx);
+    for (auto &j : s->ready)
+        if (j->ctx) erp_ctx_destroy(j->ctx);
     for (auto &kv : s->groups) kv.second.drop_graph();
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;

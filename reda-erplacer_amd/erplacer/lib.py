@@ -79,7 +79,8 @@ class JobResult(C.Structure):
 class ServerStat(C.Structure):
     _fields_ = [("pending", C.c_int64), ("active", C.c_int64), ("finished", C.c_int64),
                 ("bytes_in_use", C.c_int64), ("rounds", C.c_int64),
-                ("batched_launches", C.c_int64), ("max_batch", C.c_int64), ("clock", C.c_double)]
+                ("batched_launches", C.c_int64), ("max_batch", C.c_int64), ("clock", C.c_double),
+                ("admit_seconds", C.c_double), ("round_seconds", C.c_double), ("retire_seconds", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
