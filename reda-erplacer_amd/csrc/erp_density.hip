@@ -200,7 +200,8 @@ __global__ void __launch_bounds__(kBlock) k_sort_keys(const JobDesc *__restrict_
     __shared__ int lh[kSortTiles];
     const SortBufs &sb = jd[blockIdx.y].sb;
     const int64_t num_cells = sb.num_cells;
-    if ((int64_t)blockIdx.x * kScatterCells >= num_cells) return;
+    const int64_t nblk = (num_cells + kScatterCells - 1) / kScatterCells;
+    if ((int64_t)blockIdx.x >= nblk) return;
     const int32_t *__restrict__ perm = sb.perm;
     const float2 *__restrict__ pos = sb.pos;
     const float4 *__restrict__ sstat = sb.sstat;
@@ -211,7 +212,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_keys(const JobDesc *__restrict_
     const float inv_bw = 1.f / sb.bw, inv_bh = 1.f / sb.bh;
     for (int t = threadIdx.x; t < ntiles; t += kBlock) lh[t] = 0;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kScatterCells;
+    const int64_t base = xcd_block(blockIdx.x, nblk) * kScatterCells;
     // all loads of the thread's cells first (they are then in flight together)
     int32_t cid[kScatterPer];
     float4 stv[kScatterPer];
@@ -332,7 +333,8 @@ __global__ void __launch_bounds__(kBlock) k_sort_place(const JobDesc *__restrict
     __shared__ int lh[kSortTiles];
     const SortBufs &sb = jd[blockIdx.y].sb;
     const int64_t num_cells = sb.num_cells;
-    if ((int64_t)blockIdx.x * kScatterCells >= num_cells) return;
+    const int64_t nblk = (num_cells + kScatterCells - 1) / kScatterCells;
+    if ((int64_t)blockIdx.x >= nblk) return;
     const int32_t *__restrict__ perm = sb.perm;
     const int32_t *__restrict__ keys = sb.keys;
     const float2 *__restrict__ ll = sb.ll;
@@ -345,7 +347,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_place(const JobDesc *__restrict
     float2 *__restrict__ roff = sb.roff;
     for (int t = threadIdx.x; t < ntiles; t += kBlock) lh[t] = 0;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kScatterCells;
+    const int64_t base = xcd_block(blockIdx.x, nblk) * kScatterCells;
     int key[kScatterPer], rank[kScatterPer];
     int32_t cperm[kScatterPer];
     float2 cll[kScatterPer];

@@ -5,6 +5,17 @@
 
 namespace erp {
 
+// Workgroups are handed to the eight XCDs round robin (launch index w runs on
+// XCD w mod 8), and each XCD has its own L2.  xcd_block(w, nb) renumbers the nb
+// workgroups of one job so that every XCD processes one contiguous range of
+// logical blocks: neighbouring cells, nets and bins then share one L2 instead
+// of being fetched into all eight.  A bijection on [0, nb); the identity above.
+__device__ __forceinline__ int64_t xcd_block(int64_t w, int64_t nb) {
+    if (w >= nb) return w;
+    const int64_t q = nb >> 3, r = nb & 7, x = w & 7;
+    return x * q + (x < r ? x : r) + (w >> 3);
+}
+
 // ------------------------------------------------------------ density
 constexpr int kScatterPer = 4;                       // cells per thread
 constexpr int kScatterCells = kScatterPer * kBlock;   // cells per workgroup

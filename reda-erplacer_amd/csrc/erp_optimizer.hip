@@ -110,8 +110,10 @@ __device__ __forceinline__ float2 combine(const CellBufs &c, int64_t i, float2 g
 
 #define ERP_CELL_JOB()                                                              \
     const CellBufs &c = jd[blockIdx.y].cb;                                         \
-    if ((int64_t)blockIdx.x * kBlock >= c.C) return; /* uniform per workgroup */   \
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nblk = (c.C + kBlock - 1) / kBlock;                              \
+    if ((int64_t)blockIdx.x >= nblk) return; /* uniform per workgroup */           \
+    const int64_t blk = xcd_block(blockIdx.x, nblk);                               \
+    const int64_t i = blk * kBlock + threadIdx.x;
 
 __global__ void __launch_bounds__(kBlock) k_pin2cell(const JobDesc *__restrict__ jd) {
     ERP_CELL_JOB();
@@ -135,8 +137,8 @@ __global__ void __launch_bounds__(kBlock) k_l1(const JobDesc *__restrict__ jd) {
     a = block_sum_f64(a, red);
     b = block_sum_f64(b, red);
     if (threadIdx.x == 0) {
-        c.part_a[blockIdx.x] = a;
-        c.part_b[blockIdx.x] = b;
+        c.part_a[blk] = a;
+        c.part_b[blk] = b;
     }
 }
 
@@ -163,8 +165,8 @@ __global__ void __launch_bounds__(kBlock) k_combine(const JobDesc *__restrict__ 
     a = block_sum_f64(a, red);
     b = block_sum_f64(b, red);
     if (threadIdx.x == 0) {
-        c.part_a[blockIdx.x] = a;
-        c.part_b[blockIdx.x] = b;
+        c.part_a[blk] = a;
+        c.part_b[blk] = b;
     }
 }
 
@@ -188,8 +190,8 @@ __global__ void __launch_bounds__(kBlock) k_cell_fused(const JobDesc *__restrict
     a = block_sum_f64(a, red);
     b = block_sum_f64(b, red);
     if (threadIdx.x == 0) {
-        c.part_a[blockIdx.x] = a;
-        c.part_b[blockIdx.x] = b;
+        c.part_a[blk] = a;
+        c.part_b[blk] = b;
     }
 }
 
