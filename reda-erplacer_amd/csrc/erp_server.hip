@@ -85,11 +85,15 @@ struct Group {
     DevBuf state_dev;               // (hpwl, overflow, nonfinite) per member, written by the graph
     double *state_host = nullptr;   // pinned copy, read after each round
     int capacity = 0;
+    // the group's own stream: batches of different grid sizes are small and run
+    // side by side (non-blocking, default priority)
+    hipStream_t stream = nullptr;
     Group() = default;
     Group(const Group &) = delete;
     Group &operator=(const Group &) = delete;
     ~Group() {
         drop_graph();
+        if (stream) (void)hipStreamDestroy(stream);
         if (state_host) (void)hipHostFree(state_host);
     }
     void drop_graph() {
@@ -280,27 +284,35 @@ int rebuild(erp_server *s, Group &g) {
 int round(erp_server *s) {
     std::vector<Job *> running;
     std::vector<const double *> state;      // (hpwl, overflow, nonfinite) of running[k]
+    // rebuilds (re-uploads and re-captures) first, so that no batch is running
+    // while another group's buffers are reallocated
     for (auto &kv : s->groups) {
         Group &g = kv.second;
         if (g.dirty) {
             int rc = rebuild(s, g);
             if (rc) return rc;
         }
+    }
+    std::vector<hipStream_t> used;
+    for (auto &kv : s->groups) {
+        Group &g = kv.second;
         if (!g.graph) continue;
-        ERP_HIP_TRY(hipGraphLaunch(g.graph, s->stream));
+        if (!g.stream) ERP_HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+        ERP_HIP_TRY(hipGraphLaunch(g.graph, g.stream));
         {
             std::lock_guard<std::mutex> lk(s->mu);
             ++s->batched_launches;
         }
         ERP_HIP_TRY(hipMemcpyAsync(g.state_host, g.state_dev.p, sizeof(double) * 3 * g.members.size(),
-                                   hipMemcpyDeviceToHost, s->stream));
+                                   hipMemcpyDeviceToHost, g.stream));
+        used.push_back(g.stream);
         for (size_t k = 0; k < g.members.size(); ++k) {
             running.push_back(g.members[k]);
             state.push_back(g.state_host + 3 * k);
         }
     }
     if (running.empty()) return ERP_OK;
-    ERP_HIP_TRY(hipStreamSynchronize(s->stream));
+    for (hipStream_t st : used) ERP_HIP_TRY(hipStreamSynchronize(st));
     {
         std::lock_guard<std::mutex> lk(s->mu);
         ++s->rounds;
@@ -393,6 +405,10 @@ void worker_main(erp_server *s) {
         }
         const double tb = s->now();
         int rc = round(s);
+        if (rc) {   // batches already launched finish before their contexts are freed
+            for (auto &kv : s->groups)
+                if (kv.second.stream) (void)hipStreamSynchronize(kv.second.stream);
+        }
         std::lock_guard<std::mutex> lk(s->mu);
         s->round_s += s->now() - tb;
         if (rc) {
