@@ -125,7 +125,11 @@ struct erp_server {
     bool stop = false;
     int worker_rc = ERP_OK;
     std::string worker_error;
-    std::thread worker, admitter;
+    std::thread worker;
+    // context creation is mostly host work (net ordering, CSR, uploads): several
+    // jobs are prepared at once
+    static constexpr int kAdmitters = 4;
+    std::vector<std::thread> admitters;
     // jobs holding a slot (admitted and not finished)
     size_t resident() const { return preparing.size() + ready.size() + active.size(); }
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
@@ -459,7 +463,7 @@ int erp_server_create(const erp_server_config *cfg, erp_server **out) {
         return fail(ERP_ERR_HIP, "hipStreamCreate failed");
     }
     s->worker = std::thread(worker_main, s);
-    s->admitter = std::thread(admitter_main, s);
+    for (int k = 0; k < erp_server::kAdmitters; ++k) s->admitters.emplace_back(admitter_main, s);
     *out = s;
     return ERP_OK;
 }
@@ -576,7 +580,8 @@ int erp_server_destroy(erp_server *s) {
         s->cv.notify_all();
         s->cv_admit.notify_all();
     }
-    if (s->admitter.joinable()) s->admitter.join();   // finishes the job it is preparing
+    for (auto &t : s->admitters)      // each finishes the job it is preparing
+        if (t.joinable()) t.join();
     if (s->worker.joinable()) s->worker.join();
     (void)hipSetDevice(s->cfg.device);
     for (auto &kv : s->active)
