@@ -464,6 +464,16 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     ctx->coef_q = (int64_t)std::llrint(ctx->coef * ctx->fx_scale);
 
     // ------------------------------------------------------------- device buffers
+    // one slab for every buffer below (one hipMalloc / hipFree per context); a
+    // buffer that does not fit gets its own allocation
+    {
+        const size_t est = estimate_ctx_bytes(d, cfg);
+        if (ctx->slab_mem.alloc(est) == ERP_OK) {
+            ctx->slab.base = ctx->slab_mem.as<char>();
+            ctx->slab.cap = est;
+        }
+    }
+    SlabScope slab_scope(&ctx->slab);
 #define TRY(x) do { rc = (x); if (rc) { erp_ctx_destroy(ctx); return rc; } } while (0)
     TRY(upload(ctx->d_x, d->x, (size_t)N, s));
     TRY(upload(ctx->d_y, d->y, (size_t)N, s));

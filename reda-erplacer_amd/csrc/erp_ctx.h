@@ -21,7 +21,10 @@ struct erp_ctx {
     bool debug_parts = true;   // keep per-cell WA / density gradients in the fused loop
     bool keep_q = false;       // keep the movable accumulators after pass 1
     bool q_dirty = false;
-    // device buffers
+    // device buffers: carved from one slab sized by the admission estimate
+    // (declared first, released last)
+    DevBuf slab_mem;
+    Slab slab;
     DevBuf d_x, d_y, d_w, d_h, d_npins_all;
     DevBuf pos, prev_pos, prev_g, g, gwl, gd;
     DevBuf csize, coff, ratio, area, npins, hi;

@@ -26,9 +26,33 @@ int fail(int code, const std::string &msg);
     } while (0)
 
 // ---------------------------------------------------------- device buffers
+// A slab of device memory that buffers are carved from (256-byte aligned,
+// never freed one by one).  While a slab is active on a thread (SlabScope),
+// DevBuf::alloc takes memory from it and falls back to hipMalloc when it is
+// full: a context is created with one hipMalloc and destroyed with one hipFree.
+struct Slab {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    void *take(size_t nbytes) {
+        const size_t off = (used + 255) & ~size_t(255);
+        if (!base || off > cap || nbytes > cap - off) return nullptr;
+        used = off + nbytes;
+        return base + off;
+    }
+};
+inline thread_local Slab *tl_slab = nullptr;
+struct SlabScope {
+    Slab *prev;
+    explicit SlabScope(Slab *s) : prev(tl_slab) { tl_slab = s; }
+    ~SlabScope() { tl_slab = prev; }
+    SlabScope(const SlabScope &) = delete;
+    SlabScope &operator=(const SlabScope &) = delete;
+};
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool owned = false;          // from hipMalloc (else carved from a slab)
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -36,6 +60,13 @@ struct DevBuf {
     int alloc(size_t nbytes) {
         release();
         if (nbytes == 0) nbytes = 16;
+        if (tl_slab) {
+            if (void *q = tl_slab->take(nbytes)) {
+                p = q;
+                bytes = nbytes;
+                return ERP_OK;
+            }
+        }
         hipError_t e = hipMalloc(&p, nbytes);
         if (e != hipSuccess) {
             p = nullptr;
@@ -43,12 +74,14 @@ struct DevBuf {
                                          ") failed: " + hipGetErrorString(e));
         }
         bytes = nbytes;
+        owned = true;
         return ERP_OK;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
+        owned = false;
     }
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
