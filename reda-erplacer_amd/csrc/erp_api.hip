@@ -189,6 +189,11 @@ void drop_graphs(erp_ctx *ctx) {
 }
 
 int capture(erp_ctx *ctx, int iters, hipGraphExec_t *out) {
+    if (!ctx->side) {       // the WA branch's stream and events (contexts that never capture skip them)
+        ERP_HIP_TRY(hipStreamCreate(&ctx->side));
+        ERP_HIP_TRY(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+        ERP_HIP_TRY(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+    }
     ERP_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     int rc = ERP_OK;
     for (int k = 0; k < iters && rc == ERP_OK; ++k) rc = launch_fused_iteration(ctx, true);
@@ -432,13 +437,7 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     ctx->device = cfg->device;
     DeviceGuard g(ctx->device);
     if (g.rc) { delete ctx; return g.rc; }
-    if (hipStreamCreate(&ctx->stream) != hipSuccess || hipStreamCreate(&ctx->side) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
-        if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
-        if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
-        if (ctx->side) (void)hipStreamDestroy(ctx->side);
-        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (hipStreamCreate(&ctx->stream) != hipSuccess) {   // the side branch: at first capture
         delete ctx;
         return fail(ERP_ERR_HIP, "hipStreamCreate failed");
     }
