@@ -105,7 +105,9 @@ int phase_hpwl(erp_ctx *ctx, const JobDesc *d) {
     return launch_scalar(d, 1, SC_HPWL, ctx->stream);
 }
 
-int run_potential(erp_ctx *ctx) {
+// compute_potential (epotential.rs:225-374) in two halves: the movable density
+// (sort + scatter), then the spectral solve (density map -> field maps).
+int run_scatter(erp_ctx *ctx) {
     KernelTimer *t = ctx->timing ? ctx->timer : nullptr;
     int rc;
     if (ctx->q_dirty) {
@@ -117,13 +119,23 @@ int run_potential(erp_ctx *ctx) {
     rc = ctx->owner_scatter ? launch_scatter_owner(djob(ctx), ctx->dims, 1, ctx->stream)
                             : launch_scatter(djob(ctx), ctx->dims, 1, true, ctx->stream);
     if (t) t->end();
-    if (rc) return rc;
+    return rc;
+}
+
+int run_spectral(erp_ctx *ctx) {
+    KernelTimer *t = ctx->timing ? ctx->timer : nullptr;
+    int rc;
     if (ctx->spectral_mode == ERP_SPECTRAL_MFMA_BF16X3)
         rc = launch_potential_mfma(ctx->job.pb, ctx->mfma, ctx->stream, t);
     else
         rc = launch_potential_fft(djob(ctx), ctx->dims, 1, ctx->stream, t);
     if (ctx->keep_q) ctx->q_dirty = true;
     return rc;
+}
+
+int run_potential(erp_ctx *ctx) {
+    int rc = run_scatter(ctx);
+    return rc ? rc : run_spectral(ctx);
 }
 
 int phase_potential(erp_ctx *ctx) {
@@ -151,25 +163,29 @@ int phase_scheduler(erp_ctx *ctx) {
 }
 
 // One iteration (iteration >= 1) as a fused kernel sequence: what the graphs hold.
-// fork: (graph capture only) the WA kernel runs on a second branch beside the
-// sort / scatter / spectral chain -- both only read the positions -- and joins
-// before the cell kernel.
+// fork: (graph capture only) after the scatter, the WA kernel and the per-cell
+// sum of its pin gradients (k_pin2cell) run on a second branch beside the
+// spectral passes -- which leave most of the GPU idle (two workgroups per CU)
+// -- and join before the cell kernel, which then reads the summed gradient.
+// Both branches only read the positions; the results are bit-identical.
 int launch_fused_iteration(erp_ctx *ctx, bool fork = false) {
     KernelTimer *t = ctx->timing ? ctx->timer : nullptr;
     int rc;
     if (fork) {
+        if ((rc = run_scatter(ctx))) return rc;
         ERP_HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
         ERP_HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
         if ((rc = launch_wa(djob(ctx), ctx->dims, 1, ctx->side))) return rc;
+        if ((rc = launch_pin2cell(djob(ctx), ctx->dims, 1, ctx->side))) return rc;
         ERP_HIP_TRY(hipEventRecord(ctx->ev_join, ctx->side));
-        if ((rc = run_potential(ctx))) return rc;
+        if ((rc = run_spectral(ctx))) return rc;
         ERP_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
     } else {
         if ((rc = launch_wa(djob(ctx), ctx->dims, 1, ctx->stream, t))) return rc;
         if ((rc = run_potential(ctx))) return rc;
     }
     if (t) t->begin("cell_fused");
-    rc = launch_cell_fused(djob(ctx), ctx->dims, 1, ctx->debug_parts, ctx->stream);
+    rc = launch_cell_fused(djob(ctx), ctx->dims, 1, ctx->debug_parts, ctx->stream, fork);
     if (t) t->end();
     if (rc) return rc;
     if (t) t->begin("scalar");

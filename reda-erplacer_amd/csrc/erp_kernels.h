@@ -165,8 +165,9 @@ int launch_pin2cell(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s
 int launch_efield(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
 int launch_l1(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
 int launch_combine(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
+// read_gwl: the per-cell WA gradient was summed by k_pin2cell already
 int launch_cell_fused(const JobDesc *d, const BatchDims &bd, int nj, bool store_parts,
-                      hipStream_t s);
+                      hipStream_t s, bool read_gwl = false);
 int launch_update(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
 // out[3j..3j+2] = hpwl, overflow, nonfinite of job j
 int launch_gather_state(const JobDesc *d, int nj, double *out, hipStream_t s);

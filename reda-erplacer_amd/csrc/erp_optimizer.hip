@@ -170,19 +170,21 @@ __global__ void __launch_bounds__(kBlock) k_combine(const JobDesc *__restrict__ 
     }
 }
 
-template <bool STORE_PARTS>
+// READ_GWL: the WA gradient per cell was summed already (k_pin2cell, on the
+// graph branch that runs beside the spectral passes) and is read from gwl.
+template <bool STORE_PARTS, bool READ_GWL>
 __global__ void __launch_bounds__(kBlock) k_cell_fused(const JobDesc *__restrict__ jd) {
     __shared__ double red[kBlock / 64];
     ERP_CELL_JOB();
     const DevState *__restrict__ st = c.st;
     double a = 0.0, b = 0.0;
     if (i < c.C) {
-        const float2 gwl = pin2cell(c, i);
+        const float2 gwl = READ_GWL ? c.gwl[i] : pin2cell(c, i);
         const float2 gd = efield_cell(c, i, 1.f / c.bw, 1.f / c.bh);
         const float2 g = combine(c, i, gwl, gd, (float)st->density_weight);
         c.g[i] = g;
         if (STORE_PARTS) {
-            c.gwl[i] = gwl;
+            if (!READ_GWL) c.gwl[i] = gwl;
             c.gd[i] = gd;
         }
         bb_terms(c, i, g, a, b);
@@ -426,9 +428,13 @@ int launch_combine(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s)
     ERP_LAUNCH_CELLS(k_combine, d);
 }
 int launch_cell_fused(const JobDesc *d, const BatchDims &bd, int nj, bool store_parts,
-                      hipStream_t s) {
-    if (store_parts) ERP_LAUNCH_CELLS(k_cell_fused<true>, d);
-    ERP_LAUNCH_CELLS(k_cell_fused<false>, d);
+                      hipStream_t s, bool read_gwl) {
+    if (read_gwl) {
+        if (store_parts) ERP_LAUNCH_CELLS((k_cell_fused<true, true>), d);
+        ERP_LAUNCH_CELLS((k_cell_fused<false, true>), d);
+    }
+    if (store_parts) ERP_LAUNCH_CELLS((k_cell_fused<true, false>), d);
+    ERP_LAUNCH_CELLS((k_cell_fused<false, false>), d);
 }
 int launch_update(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s) {
     ERP_LAUNCH_CELLS(k_update, d);
