@@ -361,7 +361,10 @@ __global__ void __launch_bounds__(kBlock) k_col_spectral(const JobDesc *__restri
     float2 *tw = lds;
     float2 *dtw_s = lds + (m >> 1);   // e^{-i pi k / 2L} staged in LDS
     float2 *buf = dtw_s + m;                          // 3G complex arrays of m
-    float *acol = reinterpret_cast<float *>(buf + 3 * G * m);  // S real arrays of m
+    // S real arrays of m in the second inverse array's space (unused until the
+    // products below write the inverse inputs): 72 KB of LDS at m = 2048, two
+    // workgroups per CU
+    float *acol = reinterpret_cast<float *>(buf + G * m);
     const int v0 = strip_of_block(blockIdx.x, gridDim.x) * S;
     // the strip's column segments are loaded into registers first (all loads
     // in flight together: one workgroup per CU cannot hide them one by one)
@@ -402,8 +405,30 @@ __global__ void __launch_bounds__(kBlock) k_col_spectral(const JobDesc *__restri
         }
     }
     __syncthreads();
-    // spectral products and inverse pre-twiddles, all three kinds
-    for (int idx = threadIdx.x; idx < m * G; idx += blockDim.x) {
+    // spectral products and inverse pre-twiddles, all three kinds.  The
+    // coefficients share LDS with the inverse inputs, so every thread reads all
+    // of its coefficients before any inverse input is written.
+    constexpr int kMaxK = 8;                   // m * G / blockDim <= 8 (m <= 2048, G = 1)
+    const int nk = (m * G + (int)blockDim.x - 1) / (int)blockDim.x;
+    float ca[kMaxK][2], cr[kMaxK][2];
+#pragma unroll
+    for (int r = 0; r < kMaxK; ++r) {
+        const int idx = threadIdx.x + r * blockDim.x;
+        if (r < nk && idx < m * G) {
+            const int k = idx / G, g = idx - k * G;
+            const int kr = (m - k) & (m - 1);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                ca[r][h] = acol[(2 * g + h) * m + k];
+                cr[r][h] = k ? acol[(2 * g + h) * m + kr] : 0.f;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kMaxK; ++r) {
+        const int idx = threadIdx.x + r * blockDim.x;
+        if (r >= nk || idx >= m * G) continue;
         const int k = idx / G, g = idx - k * G;
         const int kr = (m - k) & (m - 1);
         const float2 w = dtw_s[k];
@@ -418,8 +443,8 @@ __global__ void __launch_bounds__(kBlock) k_col_spectral(const JobDesc *__restri
             float d_k = wu_k * wu_k + wv * wv, d_r = wu_r * wu_r + wv * wv;
             float inv_k = (k == 0 && v == 0) ? 0.f : 1.f / d_k;
             float inv_r = (kr == 0 && v == 0) ? 0.f : 1.f / d_r;
-            const float ak = acol[s * m + k];
-            const float ar = k ? acol[s * m + kr] : 0.f;
+            const float ak = ca[r][h];
+            const float ar = cr[r][h];
             const float wxk = wu_k * inv_k * 0.5f, wxr = wu_r * inv_r * 0.5f;
             const float wyk = wv * inv_k * 0.5f, wyr = wv * inv_r * 0.5f;
             // IDXST along x of a*Wx: reversed spectrum (k -> m-k)
@@ -637,7 +662,8 @@ int launch_potential_fft(const JobDesc *d, const BatchDims &bd, int nj, hipStrea
     // pass 2 (operator constants exactly as frequencies.rs computes them in T=f32)
     const float two_pi_over_m = (float)(2.0 * M_PI) / (float)m;
     const float two_pi_over_n = (float)(2.0 * M_PI) / (float)n;
-    size_t l2 = sizeof(float2) * ((size_t)m / 2 + (size_t)m + (size_t)3 * G * m) + sizeof(float) * (size_t)2 * G * m;
+    if ((size_t)m * G > 8 * (size_t)kBlock) return fail(ERP_ERR_UNSUPPORTED, "column pass: m * G > 2048");
+    size_t l2 = sizeof(float2) * ((size_t)m / 2 + (size_t)m + (size_t)3 * G * m);
     if ((rc = set_lds_limit((const void *)k_col_spectral, l2))) return rc;
     if (t) t->begin("pass2_col_spectral");
     hipLaunchKernelGGL(k_col_spectral, dim3(n / (2 * G), nj), dim3(kBlock), l2, s, d, logM, G,
