@@ -19,8 +19,8 @@ __device__ __forceinline__ int64_t xcd_block(int64_t w, int64_t nb) {
 // ------------------------------------------------------------ density
 constexpr int kScatterPer = 4;                       // cells per thread
 constexpr int kScatterCells = kScatterPer * kBlock;   // cells per workgroup
-constexpr int kScatterTile = 4096;                    // LDS int64 bins per workgroup
-constexpr int kScatterSide = 64;                      // tile side when the box is too big
+constexpr int kScatterTile = 8192;                    // LDS int64 bins per workgroup
+constexpr int kScatterSide = 128;                     // tile side when the box is too big
 constexpr int kSortTiles = 4096;                      // max sort tiles (64 x 64 Morton)
 constexpr int kOwnerVisits = 4096;                    // cell visits per owner-scatter workgroup
 struct SortGeom {
