@@ -514,7 +514,7 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     TRY(ctx->q_fix.alloc(sizeof(int64_t) * (size_t)ctx->B));
     for (DevBuf *b : {&ctx->rho, &ctx->rowt, &ctx->p1, &ctx->p2, &ctx->p3, &ctx->pot, &ctx->auv})
         TRY(b->alloc(sizeof(float) * (size_t)ctx->B));
-    TRY(ctx->fxy.alloc(sizeof(float2) * (size_t)ctx->B));
+    TRY(ctx->fxy.alloc(sizeof(float2) * fxy_bins(ctx->m, ctx->n)));
     TRY(ctx->state.alloc(sizeof(DevState)));
     TRY(ctx->gamma_override.alloc(sizeof(double)));
     TRY(ctx->hpwl_dbu.alloc(sizeof(unsigned long long)));
@@ -892,9 +892,13 @@ int erp_get_map(erp_ctx *ctx, int which, float *out) {
         return ERP_OK;
     }
     if (which == ERP_MAP_FIELD_X || which == ERP_MAP_FIELD_Y) {   // one float of each pair
-        const char *base = ctx->fxy.as<char>() + (which == ERP_MAP_FIELD_Y ? sizeof(float) : 0);
-        ERP_HIP_TRY(hipMemcpy2D(out, sizeof(float), base, sizeof(float2), sizeof(float),
-                                (size_t)ctx->B, hipMemcpyDeviceToHost));
+        std::vector<float2> f(fxy_bins(ctx->m, ctx->n));
+        ERP_HIP_TRY(hipMemcpy(f.data(), ctx->fxy.p, sizeof(float2) * f.size(), hipMemcpyDeviceToHost));
+        for (int k = 0; k < ctx->m; ++k)
+            for (int h = 0; h < ctx->n; ++h) {
+                const float2 v = f[fxy_index(k, h, ctx->n)];
+                out[(size_t)k * ctx->n + h] = which == ERP_MAP_FIELD_Y ? v.y : v.x;
+            }
         return ERP_OK;
     }
     const DevBuf *src = which == ERP_MAP_DENSITY ? &ctx->rho : which == ERP_MAP_AUV ? &ctx->auv
@@ -1196,13 +1200,14 @@ int erp_compute_electric_force(int64_t nc, int32_t m, int32_t n, const float *fi
     if (nc == 0) return ERP_OK;
     LeafScope L;
     if (L.rc) return L.rc;
-    const size_t B = (size_t)m * n;
     std::vector<float2> p, sz, of;
     pack2(p, x, y, nc);
     pack2(sz, cw, ch, nc);
     pack2(of, ox, oy, nc);
-    std::vector<float2> fxy;
-    pack2(fxy, field_x, field_y, (int64_t)B);
+    std::vector<float2> fxy(fxy_bins(m, n), make_float2(0.f, 0.f));   // 4 x 4-bin blocks
+    for (int k = 0; k < m; ++k)
+        for (int h = 0; h < n; ++h)
+            fxy[fxy_index(k, h, n)] = make_float2(field_x[(size_t)k * n + h], field_y[(size_t)k * n + h]);
     DevBuf dp, ds, dof, dr, dfxy, dg;
     int rc;
     if ((rc = upload(dp, p.data(), p.size(), L.s)) || (rc = upload(ds, sz.data(), sz.size(), L.s)) ||

@@ -90,6 +90,18 @@ struct DevBuf {
 constexpr int kBlock = 256;          // threads per workgroup for streaming kernels
 constexpr int kMinBins = 64, kMaxBins = 2048;
 
+// Field map layout: blocks of 4 x 4 bins, one 128-byte line each, blocks
+// row-major over the (padded) grid, both field components of a bin in one
+// float2.  A cell's window of up to 4 x 4 bins touches 1-4 lines (3-6 in a
+// row-major map).
+__host__ __device__ inline size_t fxy_index(int k, int h, int n) {
+    return ((size_t)(k >> 2) * (size_t)((n + 3) >> 2) + (size_t)(h >> 2)) * 16 +
+           (size_t)(((k & 3) << 2) | (h & 3));
+}
+__host__ __device__ inline size_t fxy_bins(int m, int n) {
+    return (size_t)((m + 3) >> 2) * (size_t)((n + 3) >> 2) * 16;
+}
+
 inline bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
 inline int ilog2(int64_t v) { int r = 0; while ((int64_t(1) << r) < v) ++r; return r; }
 
@@ -139,7 +151,7 @@ struct PotentialBufs {
     float *rho;             // [B]  density map after 1/bin_area scaling
     float *rowt;            // [B]  row-DCT intermediate
     float *p1, *p2, *p3;    // [B]  column-inverse intermediates
-    float2 *fxy;            // [B]  field (x, y) per bin
+    float2 *fxy;            // field (x, y) per bin, at fxy_index
     float *pot;             // [B]
     float *auv;             // [B] or nullptr (store DCT coefficients)
     int64_t *ovf_part;      // [m / rows-per-block] int64 overflow partials

@@ -96,7 +96,7 @@ struct CellBufs {
     float2 *prev_pos, *prev_g;  // [C]
     float2 *pos_out;            // == pos (movable part written)
     double *part_a, *part_b;    // per-workgroup FP64 partials
-    const float2 *fxy;          // field maps, (x, y) interleaved per bin
+    const float2 *fxy;          // field maps, (x, y) per bin at fxy_index (4 x 4-bin blocks)
     int m, n;
     float bw, bh;
     DevState *st;               // scalar state (density weight, alpha, beta)

@@ -330,12 +330,17 @@ __global__ void __launch_bounds__(256) k_mfma(MfmaArgs a) {
                 d[(size_t)row * n + col] = pe + po;
                 d[(size_t)(m - 1 - row) * n + col] = sine ? po - pe : pe - po;
             } else {
-                const int st = a.dst_stride[z];
-                float *d = a.dst[z] + (size_t)row * n * st;
+                const int st = a.dst_stride[z];   // 2: a component of the field map (fxy_index)
                 const float x0 = 4.f * (pe + po);
                 const float x1 = 4.f * (sine ? po - pe : pe - po);
-                d[(size_t)col * st] = x0;
-                d[(size_t)(n - 1 - col) * st] = x1;
+                if (st == 2) {
+                    a.dst[z][2 * fxy_index(row, col, n)] = x0;
+                    a.dst[z][2 * fxy_index(row, n - 1 - col, n)] = x1;
+                } else {
+                    float *d = a.dst[z] + (size_t)row * n;
+                    d[col] = x0;
+                    d[n - 1 - col] = x1;
+                }
                 if (a.energy_part && z == 2) {
                     const float *rr = a.rho + (size_t)row * n;
                     energy += (double)(x0 * rr[col]) + (double)(x1 * rr[n - 1 - col]);

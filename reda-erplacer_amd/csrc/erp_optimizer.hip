@@ -42,19 +42,27 @@ __device__ __forceinline__ float2 efield_cell(const CellBufs &c, int64_t i, floa
     int yh = (int)((ny + sz.y - 0.f) * inv_bh) + 1;
     yh = yh > c.n ? c.n : yh;
     float gx = 0.f, gy = 0.f;
-    if (yh - yl <= 4 && c.n >= 8) {
-        // a window of up to 4 bins lies in the 6 bins from the even bin at or
-        // below yl: three aligned 16-byte loads per row (both field components)
-        const int ys = min(yl & ~1, c.n - 6);
+    if (yh - yl <= 5) {
+        // the window's columns lie in the 4-bin block at or below yl and the
+        // next one: per row, two aligned 16-byte loads per block (the next
+        // block only when the window reaches it)
+        const int hb = yl & ~3;
+        const bool two = yh > hb + 4;
         for (int k = xl; k < xh; ++k) {
             const float px = tri_unclamped(nx, sz.x, k, c.bw);
-            const float4 *row = reinterpret_cast<const float4 *>(c.fxy + (size_t)k * c.n + ys);
-            const float4 v0 = row[0], v1 = row[1], v2 = row[2];
-            const float vx[6] = {v0.x, v0.z, v1.x, v1.z, v2.x, v2.z};
-            const float vy[6] = {v0.y, v0.w, v1.y, v1.w, v2.y, v2.w};
+            const float4 *b0 = reinterpret_cast<const float4 *>(c.fxy + fxy_index(k, hb, c.n));
+            const float4 v0 = b0[0], v1 = b0[1];
+            float4 v2 = make_float4(0.f, 0.f, 0.f, 0.f), v3 = v2;
+            if (two) {
+                const float4 *b1 = reinterpret_cast<const float4 *>(c.fxy + fxy_index(k, hb + 4, c.n));
+                v2 = b1[0];
+                v3 = b1[1];
+            }
+            const float vx[8] = {v0.x, v0.z, v1.x, v1.z, v2.x, v2.z, v3.x, v3.z};
+            const float vy[8] = {v0.y, v0.w, v1.y, v1.w, v2.y, v2.w, v3.y, v3.w};
 #pragma unroll
-            for (int j = 0; j < 6; ++j) {
-                const int h = ys + j;
+            for (int j = 0; j < 8; ++j) {
+                const int h = hb + j;
                 if (h >= yl && h < yh) {
                     const float area = px * tri_unclamped(ny, sz.y, h, c.bh);
                     gx += area * vx[j];
@@ -65,9 +73,8 @@ __device__ __forceinline__ float2 efield_cell(const CellBufs &c, int64_t i, floa
     } else {
         for (int k = xl; k < xh; ++k) {
             const float px = tri_unclamped(nx, sz.x, k, c.bw);
-            const float2 *row = c.fxy + (size_t)k * c.n;
             for (int h = yl; h < yh; ++h) {
-                const float2 v = row[h];
+                const float2 v = c.fxy[fxy_index(k, h, c.n)];
                 const float area = px * tri_unclamped(ny, sz.y, h, c.bh);
                 gx += area * v.x;
                 gy += area * v.y;

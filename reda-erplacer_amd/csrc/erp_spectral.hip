@@ -534,8 +534,9 @@ __global__ void __launch_bounds__(kBlock) k_row_final(const JobDesc *__restrict_
         const float2 vx = buf[pos], vy = buf[pos + (F << logN)], vp = buf[pos + (2 * F << logN)];
         const float sg = (j & 1) ? -1.f : 1.f;
         const size_t o0 = (size_t)(row_base + 2 * f) * n + j;
-        fxy[o0] = make_float2(vx.x, sg * vy.x);
-        fxy[o0 + n] = make_float2(vx.y, sg * vy.y);
+        const int row = row_base + 2 * f;
+        fxy[fxy_index(row, j, n)] = make_float2(vx.x, sg * vy.x);
+        fxy[fxy_index(row + 1, j, n)] = make_float2(vx.y, sg * vy.y);
         pot[o0] = vp.x;
         pot[o0 + n] = vp.y;
         e += (double)(vp.x * rho[o0]) + (double)(vp.y * rho[o0 + n]);
