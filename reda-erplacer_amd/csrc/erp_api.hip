@@ -322,8 +322,8 @@ size_t erp::ctx_device_bytes(const erp_ctx *c) {
                             &c->q_fix, &c->rho, &c->rowt, &c->p1, &c->p2, &c->p3, &c->fxy,
                             &c->pot, &c->auv, &c->hpwl_part, &c->ovf_part, &c->energy_part,
                             &c->part_a, &c->part_b, &c->state, &c->gamma_override, &c->hpwl_dbu,
-                            &c->keys, &c->perm, &c->perm_out, &c->hist, &c->cursor, &c->tstart, &c->items, &c->rec,
-                            &c->rratio, &c->sstat, &c->sratio, &c->sll, &c->roff, &c->djob, &c->djob_gamma, &c->tm.fft_tw, &c->tm.dct_tw,
+                            &c->keys, &c->perm, &c->hist, &c->cursor, &c->tstart, &c->items, &c->rec,
+                            &c->aux, &c->sstat, &c->sratio, &c->sll, &c->djob, &c->djob_gamma, &c->tm.fft_tw, &c->tm.dct_tw,
                             &c->tn.fft_tw, &c->tn.dct_tw};
     size_t total = mfma_plan_bytes(c->mfma);
     for (const DevBuf *b : bufs) total += b->bytes;
@@ -611,15 +611,12 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     // ------------------------------------------------------------- device setup kernels
     TRY(ctx->keys.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(C, 1)));
     TRY(ctx->perm.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(C, 1)));
-    TRY(ctx->perm_out.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(C, 1)));
     TRY(ctx->rec.alloc(sizeof(float4) * (size_t)std::max<int64_t>(C, 1)));
-    TRY(ctx->rratio.alloc(sizeof(float) * (size_t)std::max<int64_t>(C, 1)));
+    TRY(ctx->aux.alloc(sizeof(float4) * (size_t)std::max<int64_t>(C, 1)));
     TRY(ctx->sstat.alloc(sizeof(float4) * (size_t)std::max<int64_t>(C, 1)));
     TRY(ctx->sratio.alloc(sizeof(float) * (size_t)std::max<int64_t>(C, 1)));
     TRY(ctx->sll.alloc(sizeof(float2) * (size_t)std::max<int64_t>(C, 1)));
-    TRY(ctx->roff.alloc(sizeof(float2) * (size_t)std::max<int64_t>(C, 1)));
     TRY(launch_iota(ctx->perm.as<int32_t>(), C, s));
-    TRY(launch_iota(ctx->perm_out.as<int32_t>(), C, s));
     TRY(ctx->hist.alloc(sizeof(int32_t) * (size_t)kSortTiles));
     TRY(ctx->cursor.alloc(sizeof(int32_t) * (size_t)kSortTiles));
     TRY(ctx->tstart.alloc(sizeof(int32_t) * (size_t)(kSortTiles + 1)));
@@ -639,13 +636,13 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
         sb.hist = ctx->hist.as<int32_t>();
         sb.cursor = ctx->cursor.as<int32_t>();
         sb.perm = ctx->perm.as<int32_t>();
-        sb.perm_out = ctx->perm_out.as<int32_t>();
+
         sb.rec = ctx->rec.as<float4>();
-        sb.rratio = ctx->rratio.as<float>();
+        sb.aux = ctx->aux.as<float4>();
         sb.sstat = ctx->sstat.as<float4>();
         sb.sratio = ctx->sratio.as<float>();
         sb.ll = ctx->sll.as<float2>();
-        sb.roff = ctx->roff.as<float2>();
+
         sb.tstart = ctx->tstart.as<int32_t>();
         sb.items = ctx->items.as<int32_t>();
         // tiles a movable cell can reach past its own: its clamped size in bins
@@ -1115,10 +1112,12 @@ int erp_compute_density_map(int64_t nc, const float *x, const float *y, const fl
     pack2(p, x, y, nc);
     pack2(sz, cw, ch, nc);
     pack2(of, ox, oy, nc);
+    std::vector<float4> aux((size_t)nc);            // only the ratio is read (no order copy)
+    for (int64_t i = 0; i < nc; ++i) aux[(size_t)i] = make_float4(0.f, 0.f, ratio[i], 0.f);
     DevBuf dp, ds, dof, dr, dq, dinit, dmap, drec;
     int rc;
     if ((rc = upload(dp, p.data(), p.size(), L.s)) || (rc = upload(ds, sz.data(), sz.size(), L.s)) ||
-        (rc = upload(dof, of.data(), of.size(), L.s)) || (rc = upload(dr, ratio, (size_t)nc, L.s)) ||
+        (rc = upload(dof, of.data(), of.size(), L.s)) || (rc = upload(dr, aux.data(), aux.size(), L.s)) ||
         (rc = dq.alloc(sizeof(int64_t) * B)) || (rc = dmap.alloc(sizeof(float) * B)) ||
         (rc = drec.alloc(sizeof(float4) * (size_t)std::max<int64_t>(nc, 1))))
         return rc;
@@ -1130,7 +1129,7 @@ int erp_compute_density_map(int64_t nc, const float *x, const float *y, const fl
     JobDesc j{};
     j.sb.num_cells = nc;
     j.sb.rec = drec.as<float4>();
-    j.sb.rratio = dr.as<float>();
+    j.sb.aux = dr.as<float4>();
     j.sb.m = m;
     j.sb.n = n;
     j.sb.bw = bw;

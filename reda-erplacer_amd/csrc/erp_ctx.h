@@ -32,7 +32,7 @@ struct erp_ctx {
     DevBuf q_mov, q_fix, rho, rowt, p1, p2, p3, fxy, pot, auv;
     DevBuf hpwl_part, ovf_part, energy_part, part_a, part_b;
     DevBuf state, gamma_override, hpwl_dbu;
-    DevBuf keys, perm, perm_out, hist, cursor, tstart, items, rec, rratio, sstat, sratio, sll, roff;
+    DevBuf keys, perm, hist, cursor, tstart, items, rec, aux, sstat, sratio, sll;
     bool owner_scatter = true;     // every movable cell reaches at most 2 tiles past its own
     SpecTables tm, tn;
     MfmaPlan *mfma = nullptr;

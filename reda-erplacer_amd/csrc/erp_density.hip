@@ -51,7 +51,8 @@ __device__ __forceinline__ void add_q(int64_t *q, int64_t v) {
 // ------------------------------------------------------------------ movable scatter
 // Cells are processed in a spatially sorted order, 1024 per workgroup.  rec
 // holds each cell's clamped lower-left corner (x + ox, y + oy) and clamped size,
-// rratio its ratio, both in that order (written by the sort below).  A
+// aux its clamp offset, ratio and id, both in that order (written by the sort
+// below).  A
 // workgroup accumulates into an LDS int64 tile of kScatterTile bins placed over
 // its cells; each touched tile bin then costs one global atomic per workgroup
 // instead of one per contribution.  A cell whose window does not fit the tile
@@ -66,10 +67,8 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const JobDesc *__restrict__ 
     const int64_t base = (int64_t)blockIdx.x * kScatterCells;
     if (base >= num_cells) return;                      // this job needs fewer workgroups
     const float4 *__restrict__ rec = sb.rec;
-    const float *__restrict__ rratio = sb.rratio;
-    const int32_t *__restrict__ perm_src = copy_perm ? sb.perm_out : nullptr;
+    const float4 *__restrict__ aux = sb.aux;
     int32_t *__restrict__ perm_dst = copy_perm ? sb.perm : nullptr;
-    const float2 *__restrict__ roff = sb.roff;
     float4 *__restrict__ sstat = sb.sstat;
     float *__restrict__ sratio = sb.sratio;
     int64_t *__restrict__ q = jd[blockIdx.y].pb.q_mov;
@@ -91,12 +90,12 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const JobDesc *__restrict__ 
             yi[j] = v.y;
             sx[j] = v.z;
             sy[j] = v.w;
-            r[j] = rratio[i];
+            const float4 a = aux[i];
+            r[j] = a.z;
             if (perm_dst) {          // the sorted order becomes the next sort's input order
-                perm_dst[i] = perm_src[i];
-                const float2 o = roff[i];
-                sstat[i] = make_float4(v.z, v.w, o.x, o.y);
-                sratio[i] = r[j];
+                perm_dst[i] = __float_as_int(a.w);
+                sstat[i] = make_float4(v.z, v.w, a.x, a.y);
+                sratio[i] = a.z;
             }
             bin_range(xi[j], sx[j], inv_bw, m, bxl[j], bxh[j]);
             bin_range(yi[j], sy[j], inv_bh, n, byl[j], byh[j]);
@@ -341,10 +340,9 @@ __global__ void __launch_bounds__(kBlock) k_sort_place(const JobDesc *__restrict
     const float4 *__restrict__ sstat = sb.sstat;
     const float *__restrict__ sratio = sb.sratio;
     int32_t *__restrict__ cursor = sb.cursor;
-    int32_t *__restrict__ perm_out = sb.perm_out;
+
     float4 *__restrict__ rec = sb.rec;
-    float *__restrict__ rratio = sb.rratio;
-    float2 *__restrict__ roff = sb.roff;
+    float4 *__restrict__ aux = sb.aux;
     for (int t = threadIdx.x; t < ntiles; t += kBlock) lh[t] = 0;
     __syncthreads();
     const int64_t base = xcd_block(blockIdx.x, nblk) * kScatterCells;
@@ -389,10 +387,8 @@ __global__ void __launch_bounds__(kBlock) k_sort_place(const JobDesc *__restrict
     for (int j = 0; j < kScatterPer; ++j) {
         if (key[j] >= 0) {
             const int dst = lh[key[j]] + rank[j];
-            perm_out[dst] = cperm[j];
             rec[dst] = make_float4(cll[j].x, cll[j].y, cst[j].x, cst[j].y);
-            rratio[dst] = crat[j];
-            roff[dst] = make_float2(cst[j].z, cst[j].w);
+            aux[dst] = make_float4(cst[j].z, cst[j].w, crat[j], __int_as_float(cperm[j]));
         }
     }
 }
@@ -445,7 +441,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter_owner(const JobDesc *__restr
     for (int k = 0; k < W * W; ++k) total += src_cnt[k];
     const int v0 = chunk * kOwnerVisits, v1 = min(v0 + kOwnerVisits, total);
     const float4 *__restrict__ rec = sb.rec;
-    const float *__restrict__ rratio = sb.rratio;
+    const float4 *__restrict__ aux = sb.aux;
     const float bw = sb.bw, bh = sb.bh;
     const float inv_bw = 1.f / bw, inv_bh = 1.f / bh;
     const double scale = jd[blockIdx.y].pb.scale;
@@ -454,11 +450,11 @@ __global__ void __launch_bounds__(kBlock) k_scatter_owner(const JobDesc *__restr
         while (off >= src_cnt[k]) { off -= src_cnt[k]; ++k; }
         const int i = src_start[k] + off;
         const float4 r4 = rec[i];
-        const float r = rratio[i];
+        const float4 a4 = aux[i];
+        const float r = a4.z;
         if (k == 0) {                               // the cell's own tile
-            sb.perm[i] = sb.perm_out[i];
-            const float2 o = sb.roff[i];
-            sb.sstat[i] = make_float4(r4.z, r4.w, o.x, o.y);
+            sb.perm[i] = __float_as_int(a4.w);
+            sb.sstat[i] = make_float4(r4.z, r4.w, a4.x, a4.y);
             sb.sratio[i] = r;
         }
         int bxl, bxh, byl, byh;

@@ -34,16 +34,17 @@ struct SortBufs {
     int m, n;
     float bw, bh;
     int32_t *perm;       // [C] current order (input of the sort)
-    int32_t *perm_out;   // [C] sorted order (copied back into perm by the scatter)
     int32_t *keys, *hist, *cursor;   // keys [C], hist/cursor [kSortTiles]
-    float4 *rec;         // [C] sorted scatter records
-    float *rratio;       // [C]
+    float4 *rec;         // [C] sorted scatter records {clamped lower-left, clamped size}
+    // [C] sorted side records {clamp offset x, y, ratio, cell id (int bits)}: one
+    // 16-byte write per cell in the sort (the cell id is copied back into perm
+    // by the scatter)
+    float4 *aux;
     // per-cell constants carried along the order so that the sort reads them
     // sequentially: {clamped w, h, offset x, y} and ratio in the order of perm
     float4 *sstat;
     float *sratio;
     float2 *ll;          // [C] clamped lower-left corners in the order of perm (sort temp)
-    float2 *roff;        // [C] clamp offsets in sorted order
     // owner scatter: first sorted index of every tile ([ntiles + 1], Morton
     // order) and the workgroup items per tile (exclusive prefix, [ntiles + 1])
     int32_t *tstart, *items;
@@ -155,7 +156,8 @@ int launch_wa(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
               KernelTimer *t = nullptr);
 int launch_sort(const JobDesc *d, const BatchDims &bd, int nj, bool owner_items, hipStream_t s,
                 KernelTimer *t = nullptr);   // owner_items: also build the owner-scatter work list
-// perm_out -> perm is copied by the scatter when copy_perm is set
+// copy_perm: the sorted cell ids (aux.w) are copied back into perm, and the
+// per-cell constants into sstat / sratio, as the next sort's input order
 int launch_scatter(const JobDesc *d, const BatchDims &bd, int nj, bool copy_perm, hipStream_t s);
 // Density scatter of the sorted cells, one workgroup per bin tile (plain stores
 // of the tile's bins; atomics only for tiles split over several workgroups), and
