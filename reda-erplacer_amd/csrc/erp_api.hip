@@ -301,7 +301,7 @@ size_t erp::estimate_ctx_bytes(const erp_design *d, const erp_config *cfg) {
     const size_t N = (size_t)d->num_nodes, C = (size_t)d->num_movable, P = (size_t)d->num_pins_total;
     const size_t E = (size_t)d->num_nets, B = (size_t)cfg->num_bins_x * (size_t)cfg->num_bins_y;
     const size_t per_node = 5 * 4 + 3 * 8 + 4;                       // DB copies, pos/csize/coff, ratio
-    const size_t per_cell = 6 * 8 + 2 * 4 + 8 + 4 * 4 + 16 + 4 + 4 + 36;  // state, area/pins, hi, sort
+    const size_t per_cell = 6 * 8 + 2 * 4 + 8 + 4 * 4 + 16 + 4 + 4 + 36 + 32;  // state, area/pins, hi, sort, packed consts
     const size_t per_pin = 4 + 8 + 8 + 4;                             // slots, grads, cell->slot
     const size_t per_bin = 2 * 8 + 9 * 4;                             // q_mov/q_fix + float maps
     const size_t partials = 16 * (P / 64 + C / 128 + (size_t)cfg->num_bins_x) + 64 * 1024;
@@ -317,7 +317,7 @@ size_t erp::estimate_ctx_bytes(const erp_design *d, const erp_config *cfg) {
 size_t erp::ctx_device_bytes(const erp_ctx *c) {
     const DevBuf *bufs[] = {&c->d_x, &c->d_y, &c->d_w, &c->d_h, &c->d_npins_all, &c->pos,
                             &c->prev_pos, &c->prev_g, &c->g, &c->gwl, &c->gd, &c->csize, &c->coff,
-                            &c->ratio, &c->area, &c->npins, &c->hi, &c->slot_node, &c->slot_off,
+                            &c->ratio, &c->area, &c->npins, &c->hi, &c->cgeo, &c->cprop, &c->slot_node, &c->slot_off,
                             &c->net_start, &c->slot_grad, &c->cstart, &c->cslots, &c->q_mov,
                             &c->q_fix, &c->rho, &c->rowt, &c->p1, &c->p2, &c->p3, &c->fxy,
                             &c->pot, &c->auv, &c->hpwl_part, &c->ovf_part, &c->energy_part,
@@ -508,6 +508,8 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     for (DevBuf *b : {&ctx->prev_pos, &ctx->prev_g, &ctx->g, &ctx->gwl, &ctx->gd, &ctx->hi})
         TRY(b->alloc(f2 * (size_t)std::max<int64_t>(C, 1)));
     TRY(ctx->area.alloc(sizeof(float) * (size_t)std::max<int64_t>(C, 1)));
+    TRY(ctx->cgeo.alloc(sizeof(float4) * (size_t)std::max<int64_t>(C, 1)));
+    TRY(ctx->cprop.alloc(sizeof(float4) * (size_t)std::max<int64_t>(C, 1)));
     TRY(ctx->npins.alloc(sizeof(float) * (size_t)std::max<int64_t>(C, 1)));
     TRY(ctx->slot_grad.alloc(f2 * (size_t)std::max<int64_t>(P, 1)));
     TRY(ctx->q_mov.alloc(sizeof(int64_t) * (size_t)ctx->B));
@@ -576,6 +578,8 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     cb.ratio = ctx->ratio.as<float>();
     cb.area = ctx->area.as<float>();
     cb.npins = ctx->npins.as<float>();
+    cb.cgeo = ctx->cgeo.as<float4>();
+    cb.cprop = ctx->cprop.as<float4>();
     cb.hi = ctx->hi.as<float2>();
     cb.cstart = ctx->cstart.as<int32_t>();
     cb.cslots = ctx->cslots.as<int32_t>();
@@ -673,6 +677,9 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
                            N, C, ctx->bw, ctx->bh, ctx->t, die_w, die_h, ctx->csize.as<float2>(),
                            ctx->coff.as<float2>(), ctx->ratio.as<float>(), ctx->area.as<float>(),
                            ctx->npins.as<float>(), ctx->hi.as<float2>(), s));
+    TRY(launch_pack_cell_consts(ctx->csize.as<float2>(), ctx->coff.as<float2>(), ctx->ratio.as<float>(),
+                                ctx->npins.as<float>(), ctx->area.as<float>(), C, ctx->cgeo.as<float4>(),
+                                ctx->cprop.as<float4>(), s));
     TRY(launch_init_sort_static(ctx->csize.as<float2>(), ctx->coff.as<float2>(), ctx->ratio.as<float>(),
                                 C, ctx->sstat.as<float4>(), ctx->sratio.as<float>(), s));
     TRY(reset_ctx(ctx));
@@ -1207,10 +1214,16 @@ int erp_compute_electric_force(int64_t nc, int32_t m, int32_t n, const float *fi
     for (int k = 0; k < m; ++k)
         for (int h = 0; h < n; ++h)
             fxy[fxy_index(k, h, n)] = make_float2(field_x[(size_t)k * n + h], field_y[(size_t)k * n + h]);
-    DevBuf dp, ds, dof, dr, dfxy, dg;
+    std::vector<float4> geo((size_t)nc), prop((size_t)nc);   // the cell kernels' packed constants
+    for (int64_t i = 0; i < nc; ++i) {
+        geo[(size_t)i] = make_float4(ox[i], oy[i], cw[i], ch[i]);
+        prop[(size_t)i] = make_float4(ratio[i], 0.f, 0.f, 0.f);
+    }
+    DevBuf dp, ds, dof, dr, dfxy, dg, dgeo, dprop;
     int rc;
     if ((rc = upload(dp, p.data(), p.size(), L.s)) || (rc = upload(ds, sz.data(), sz.size(), L.s)) ||
         (rc = upload(dof, of.data(), of.size(), L.s)) || (rc = upload(dr, ratio, (size_t)nc, L.s)) ||
+        (rc = upload(dgeo, geo.data(), geo.size(), L.s)) || (rc = upload(dprop, prop.data(), prop.size(), L.s)) ||
         (rc = upload(dfxy, fxy.data(), fxy.size(), L.s)) || (rc = dg.alloc(sizeof(float2) * (size_t)nc)))
         return rc;
     CellBufs c{};
@@ -1219,6 +1232,8 @@ int erp_compute_electric_force(int64_t nc, int32_t m, int32_t n, const float *fi
     c.csize = ds.as<float2>();
     c.coff = dof.as<float2>();
     c.ratio = dr.as<float>();
+    c.cgeo = dgeo.as<float4>();
+    c.cprop = dprop.as<float4>();
     c.gd = dg.as<float2>();
     c.fxy = dfxy.as<float2>();
     c.m = m;

@@ -27,7 +27,7 @@ struct erp_ctx {
     Slab slab;
     DevBuf d_x, d_y, d_w, d_h, d_npins_all;
     DevBuf pos, prev_pos, prev_g, g, gwl, gd;
-    DevBuf csize, coff, ratio, area, npins, hi;
+    DevBuf csize, coff, ratio, area, npins, hi, cgeo, cprop;
     DevBuf slot_node, slot_off, net_start, slot_grad, cstart, cslots;
     DevBuf q_mov, q_fix, rho, rowt, p1, p2, p3, fxy, pot, auv;
     DevBuf hpwl_part, ovf_part, energy_part, part_a, part_b;

@@ -90,6 +90,9 @@ struct CellBufs {
     const float2 *csize, *coff;
     const float *ratio;
     const float *area, *npins;  // [C] w*h and num_pins
+    // [C] the same constants packed for the cell kernels (one 16-byte load each):
+    // {clamp offset x, y, clamped w, h} and {ratio, num_pins, w*h, 0}
+    const float4 *cgeo, *cprop;
     const float2 *hi;           // [C] (W - w - 1, H - h - 1)
     const int32_t *cstart, *cslots;   // cell -> pin slots CSR (macro_2_pins order)
     const float2 *slot_grad;
@@ -124,6 +127,9 @@ int launch_reset_state(DevState *st, double gamma0, double dw0, hipStream_t s);
 int launch_init_positions(float2 *pos, const float *w, const float *h, const float *x,
                           const float *y, int64_t num_nodes, int64_t num_movable, float die_w,
                           float die_h, float2 *prev_pos, float2 *prev_g, hipStream_t s);
+int launch_pack_cell_consts(const float2 *csize, const float2 *coff, const float *ratio,
+                            const float *npins, const float *area, int64_t num_movable,
+                            float4 *cgeo, float4 *cprop, hipStream_t s);
 int launch_setup_cells(const float *w, const float *h, const float *npins, int64_t num_nodes,
                        int64_t num_movable, float bw, float bh, float target_density,
                        float die_w, float die_h, float2 *csize, float2 *coff, float *ratio,

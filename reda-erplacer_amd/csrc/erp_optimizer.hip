@@ -31,7 +31,9 @@ __device__ __forceinline__ float tri_unclamped(float x, float s, int k, float bs
 // bins [max(trunc(x/bw), 0), min(trunc((x+s)/bw) + 1, M)).
 __device__ __forceinline__ float2 efield_cell(const CellBufs &c, int64_t i, float inv_bw,
                                               float inv_bh) {
-    const float2 p = c.pos[i], o = c.coff[i], sz = c.csize[i];
+    const float2 p = c.pos[i];
+    const float4 gq = c.cgeo[i];
+    const float2 o = make_float2(gq.x, gq.y), sz = make_float2(gq.z, gq.w);
     const float nx = p.x + o.x, ny = p.y + o.y;
     int xl = (int)((nx - 0.f) * inv_bw);
     xl = xl < 0 ? 0 : xl;
@@ -81,7 +83,7 @@ __device__ __forceinline__ float2 efield_cell(const CellBufs &c, int64_t i, floa
             }
         }
     }
-    const float r = c.ratio[i];
+    const float r = c.cprop[i].x;
     return make_float2(-gx * r, -gy * r);
 }
 
@@ -111,7 +113,8 @@ __device__ __forceinline__ float2 pin2cell(const CellBufs &c, int64_t i) {
 // nesterov.rs:75-95: preconditioned combined gradient
 __device__ __forceinline__ float2 combine(const CellBufs &c, int64_t i, float2 gwl, float2 gd,
                                           float dw) {
-    const float inv_p = 1.f / fminf(c.npins[i] + dw * c.area[i], 1.f);
+    const float4 pr = c.cprop[i];
+    const float inv_p = 1.f / fminf(pr.y + dw * pr.z, 1.f);
     return make_float2((gwl.x + dw * gd.x) * inv_p, (gwl.y + dw * gd.y) * inv_p);
 }
 
@@ -471,6 +474,29 @@ int launch_init_positions(float2 *pos, const float *w, const float *h, const flo
                           float die_h, float2 *prev_pos, float2 *prev_g, hipStream_t s) {
     hipLaunchKernelGGL(k_init_positions, dim3(grid_for(num_nodes)), dim3(kBlock), 0, s, pos, w, h,
                        x, y, num_nodes, num_movable, die_w, die_h, prev_pos, prev_g);
+    ERP_HIP_TRY(hipGetLastError());
+    return ERP_OK;
+}
+
+namespace {
+__global__ void __launch_bounds__(kBlock) k_pack_cell_consts(
+    const float2 *__restrict__ csize, const float2 *__restrict__ coff, const float *__restrict__ ratio,
+    const float *__restrict__ npins, const float *__restrict__ area, int64_t num_movable,
+    float4 *__restrict__ cgeo, float4 *__restrict__ cprop) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num_movable) return;
+    const float2 o = coff[i], sz = csize[i];
+    cgeo[i] = make_float4(o.x, o.y, sz.x, sz.y);
+    cprop[i] = make_float4(ratio[i], npins[i], area[i], 0.f);
+}
+}  // namespace
+
+int launch_pack_cell_consts(const float2 *csize, const float2 *coff, const float *ratio,
+                            const float *npins, const float *area, int64_t num_movable,
+                            float4 *cgeo, float4 *cprop, hipStream_t s) {
+    if (num_movable <= 0) return ERP_OK;
+    hipLaunchKernelGGL(k_pack_cell_consts, dim3(grid_for(num_movable)), dim3(kBlock), 0, s, csize,
+                       coff, ratio, npins, area, num_movable, cgeo, cprop);
     ERP_HIP_TRY(hipGetLastError());
     return ERP_OK;
 }
