@@ -45,7 +45,8 @@ enum {
 /* Maps for erp_get_map (row-major [bx * num_bins_y + by], x-bin major). */
 enum {
     ERP_MAP_DENSITY = 0,   /* density_map after the 1/bin_area scaling (epotential.rs:284-285) */
-    ERP_MAP_AUV = 1,       /* DCT coefficients a_uv (epotential.rs:287-297) */
+    ERP_MAP_AUV = 1,       /* DCT coefficients a_uv (epotential.rs:287-297); kept only
+                              while gradients are stored (erp_set_store_gradients) */
     ERP_MAP_FIELD_X = 2,   /* field_map_x (epotential.rs:310-319) */
     ERP_MAP_FIELD_Y = 3,   /* field_map_y (epotential.rs:320-329) */
     ERP_MAP_POTENTIAL = 4, /* potential_map (epotential.rs:333-343) */
@@ -147,8 +148,9 @@ int erp_get_density_fixed(erp_ctx *ctx, int64_t *q_mov, int64_t *q_fix, double *
  * erp_get_density_fixed can return them (off by default: the loop clears them
  * in the same pass that reads them). */
 int erp_set_keep_density(erp_ctx *ctx, int keep);
-/* Keep the per-cell wirelength and density gradients in the fused loop
- * (on by default; off saves 16 bytes per cell per iteration). */
+/* Keep the per-cell wirelength and density gradients and the a_uv map in the
+ * fused loop (on by default; off saves 16 bytes per cell and a strided write of
+ * the whole a_uv map per iteration). */
 int erp_set_store_gradients(erp_ctx *ctx, int store);
 /* Clamped sizes/offsets/ratios of PotentialComputation::new (epotential.rs:83-103),
  * length num_nodes; scalars = {bin_w, bin_h, target_density, fx_scale}. */

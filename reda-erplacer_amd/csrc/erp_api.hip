@@ -953,9 +953,12 @@ int erp_set_store_gradients(erp_ctx *ctx, int store) {
     if (!ctx) return fail(ERP_ERR_INVALID, "null context");
     DeviceGuard g(ctx->device);
     if (g.rc) return g.rc;
-    if (ctx->debug_parts != (store != 0)) drop_graphs(ctx);
+    if (ctx->debug_parts == (store != 0)) return ERP_OK;
+    drop_graphs(ctx);
     ctx->debug_parts = store != 0;
-    return ERP_OK;
+    // the spectral pass writes the a_uv map only while the parts are kept
+    ctx->job.pb.auv = ctx->debug_parts ? ctx->auv.as<float>() : nullptr;
+    return upload_job(ctx);
 }
 
 int erp_get_setup(erp_ctx *ctx, float *cw, float *ch, float *ox, float *oy, float *ratio,
