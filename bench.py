@@ -209,6 +209,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="large")
     ap.add_argument("--spectral", default="fft", choices=["fft", "mfma"])
+    ap.add_argument("--skip-potential", action="store_true",
+                    help="trajectory only: skip the potential map and energy, which "
+                         "Placement::step discards (placement.rs:185); not the default")
     ap.add_argument("--timing-steps", type=int, default=0,
                     help="iterations re-run with per-kernel HIP events for the roofline "
                          "(default: the same number as --steps)")
@@ -247,6 +250,8 @@ def main():
     mode = L.SPECTRAL_FFT if args.spectral == "fft" else L.SPECTRAL_MFMA_BF16X3
     P = Placement(d, grid, target_density=td, spectral_mode=mode)
     P.set_store_gradients(False)
+    if args.skip_potential:
+        P.set_compute_potential(False)
     if args.warmup:
         P.run(args.warmup)
     P.reset()
@@ -320,6 +325,7 @@ def main():
         "config": {"workload": f"{args.workload}: {d.num_movable} cells, {d.num_nets} nets, "
                                f"{d.num_pin_total} pins, {grid}x{grid} bins",
                    "iterations_of_workload": iters_cfg, "spectral": args.spectral,
+                   "potential": "skipped (trajectory only)" if args.skip_potential else "computed",
                    "parallelism": "single GPU"},
         "final": {"hpwl": info["hpwl"], "overflow": info["overflow"], "hpwl_dbu": hpwl_dbu,
                   "iteration": info["iteration"]},

@@ -152,6 +152,12 @@ int erp_set_keep_density(erp_ctx *ctx, int keep);
  * fused loop (on by default; off saves 16 bytes per cell and a strided write of
  * the whole a_uv map per iteration). */
 int erp_set_store_gradients(erp_ctx *ctx, int store);
+/* Compute the potential map and the energy every step (on by default, as
+ * compute_potential does, epotential.rs:333-346).  Placement::step discards
+ * both (placement.rs:185), so off ("trajectory only") skips one of the three
+ * inverse transforms with an unchanged trajectory; energy then reads 0 and the
+ * potential map is not refreshed. */
+int erp_set_compute_potential(erp_ctx *ctx, int on);
 /* Clamped sizes/offsets/ratios of PotentialComputation::new (epotential.rs:83-103),
  * length num_nodes; scalars = {bin_w, bin_h, target_density, fx_scale}. */
 int erp_get_setup(erp_ctx *ctx, float *cw, float *ch, float *ox, float *oy, float *ratio,

@@ -565,6 +565,7 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     pb.bw = ctx->bw;
     pb.bh = ctx->bh;
     pb.keep_q = false;
+    pb.potential = 1;
     pb.scale = ctx->fx_scale;
 
     ctx->n_cell_part = cell_partials(C);
@@ -958,6 +959,17 @@ int erp_set_store_gradients(erp_ctx *ctx, int store) {
     ctx->debug_parts = store != 0;
     // the spectral pass writes the a_uv map only while the parts are kept
     ctx->job.pb.auv = ctx->debug_parts ? ctx->auv.as<float>() : nullptr;
+    return upload_job(ctx);
+}
+
+int erp_set_compute_potential(erp_ctx *ctx, int on) {
+    if (!ctx) return fail(ERP_ERR_INVALID, "null context");
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    if ((ctx->job.pb.potential != 0) == (on != 0)) return ERP_OK;
+    drop_graphs(ctx);
+    ctx->job.pb.potential = on ? 1 : 0;
+    ctx->job.sa.n_energy = on ? ctx->n_energy_part : 0;   // energy reads 0 while skipped
     return upload_job(ctx);
 }
 

@@ -162,6 +162,10 @@ struct PotentialBufs {
     float inv_bin_area;     // T::one() / bin_area in f32
     float bw, bh;
     bool keep_q;            // leave q_mov intact (parity inspection)
+    // 1: the third inverse (potential map) and the energy are computed
+    // (epotential.rs:333-346); 0: skipped -- Placement::step discards them
+    // (placement.rs:185), the trajectory is unchanged
+    int potential;
     double scale;           // S of the fixed-point accumulators
     const float2 *tm_fft, *tm_dct, *tn_fft, *tn_dct;   // spectral tables (length m / n)
 };

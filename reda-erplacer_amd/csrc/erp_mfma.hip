@@ -577,7 +577,9 @@ int launch_potential_mfma(const PotentialBufs &b, MfmaPlan *p, hipStream_t s, Ke
     float *out[3] = {fxy, fxy + 1, b.pot};
     const int stride[3] = {2, 2, 1};
     // xi_x: sine along x, cosine along y; xi_y: cosine along x, sine along y; phi: cosines
-    return inverse(p, X, Y, out, stride, 3, /*col_sine=*/1, /*row_sine=*/2, b.rho, b.energy_part, s, t);
+    // the potential (third inverse) and its energy only when requested
+    return inverse(p, X, Y, out, stride, b.potential ? 3 : 2, /*col_sine=*/1, /*row_sine=*/2, b.rho,
+                   b.potential ? b.energy_part : nullptr, s, t);
 }
 
 // Leaf transforms (host-facing entry points run these on device buffers).

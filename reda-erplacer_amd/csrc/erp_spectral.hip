@@ -353,6 +353,7 @@ __global__ void __launch_bounds__(kBlock) k_col_spectral(const JobDesc *__restri
     float *__restrict__ P2 = pb.p2;
     float *__restrict__ P3 = pb.p3;
     float *__restrict__ auv_out = pb.auv;
+    const int ninv = pb.potential ? 3 : 2;         // inverses: field x, field y (, potential)
     const int m = pb.m, n = pb.n;
     const float2 *__restrict__ fft_tw = pb.tm_fft;
     const float2 *__restrict__ dct_tw = pb.tm_dct;
@@ -456,10 +457,10 @@ __global__ void __launch_bounds__(kBlock) k_col_spectral(const JobDesc *__restri
         const int pos = brev(k, logM);
         buf[((0 * G + g) << logM) + pos] = pack2(vx[0], vx[1]);
         buf[((1 * G + g) << logM) + pos] = pack2(vy[0], vy[1]);
-        buf[((2 * G + g) << logM) + pos] = pack2(vp[0], vp[1]);
+        if (ninv == 3) buf[((2 * G + g) << logM) + pos] = pack2(vp[0], vp[1]);
     }
-    fft_stages<true>(buf, 3 * G, logM, tw);
-    for (int idx = threadIdx.x; idx < 3 * m * S; idx += blockDim.x) {
+    fft_stages<true>(buf, ninv * G, logM, tw);
+    for (int idx = threadIdx.x; idx < ninv * m * S; idx += blockDim.x) {
         const int kind = idx / (m * S);
         const int rem = idx - kind * m * S;
         const int i = rem / S, s = rem - i * S;
@@ -484,6 +485,7 @@ __global__ void __launch_bounds__(kBlock) k_row_final(const JobDesc *__restrict_
     float2 *__restrict__ fxy = pb.fxy;
     float *__restrict__ pot = pb.pot;
     double *__restrict__ energy_part = pb.energy_part;
+    const bool pot_on = pb.potential != 0;   // else: field maps only (energy partials 0)
     const int n = pb.n;
     const float2 *__restrict__ fft_tw = pb.tn_fft;
     const float2 *__restrict__ dct_tw = pb.tn_dct;
@@ -508,8 +510,10 @@ __global__ void __launch_bounds__(kBlock) k_row_final(const JobDesc *__restrict_
                 q[r][1] = spec_pair<INV_IDCT>(P1 + r0 + n, k, n, 1);
                 q[r][2] = spec_pair<INV_IDXST>(P2 + r0, k, n, 1);
                 q[r][3] = spec_pair<INV_IDXST>(P2 + r0 + n, k, n, 1);
-                q[r][4] = spec_pair<INV_IDCT>(P3 + r0, k, n, 1);
-                q[r][5] = spec_pair<INV_IDCT>(P3 + r0 + n, k, n, 1);
+                if (pot_on) {
+                    q[r][4] = spec_pair<INV_IDCT>(P3 + r0, k, n, 1);
+                    q[r][5] = spec_pair<INV_IDCT>(P3 + r0 + n, k, n, 1);
+                }
             }
         }
 #pragma unroll
@@ -522,24 +526,28 @@ __global__ void __launch_bounds__(kBlock) k_row_final(const JobDesc *__restrict_
             buf[pos] = pack2(inv_twiddle(q[r][0].x, q[r][0].y, w), inv_twiddle(q[r][1].x, q[r][1].y, w));
             buf[pos + (F << logN)] = pack2(inv_twiddle(q[r][2].x, q[r][2].y, w),
                                            inv_twiddle(q[r][3].x, q[r][3].y, w));
-            buf[pos + (2 * F << logN)] = pack2(inv_twiddle(q[r][4].x, q[r][4].y, w),
-                                               inv_twiddle(q[r][5].x, q[r][5].y, w));
+            if (pot_on)
+                buf[pos + (2 * F << logN)] = pack2(inv_twiddle(q[r][4].x, q[r][4].y, w),
+                                                   inv_twiddle(q[r][5].x, q[r][5].y, w));
         }
     }
-    fft_stages<true>(buf, 3 * F, logN, tw);
+    fft_stages<true>(buf, (pot_on ? 3 : 2) * F, logN, tw);
     double e = 0.0;
     for (int idx = threadIdx.x; idx < (F << logN); idx += blockDim.x) {
         const int f = idx >> logN, j = idx & (n - 1);
         const int pos = (f << logN) + perm(j, n);
-        const float2 vx = buf[pos], vy = buf[pos + (F << logN)], vp = buf[pos + (2 * F << logN)];
+        const float2 vx = buf[pos], vy = buf[pos + (F << logN)];
         const float sg = (j & 1) ? -1.f : 1.f;
-        const size_t o0 = (size_t)(row_base + 2 * f) * n + j;
         const int row = row_base + 2 * f;
         fxy[fxy_index(row, j, n)] = make_float2(vx.x, sg * vy.x);
         fxy[fxy_index(row + 1, j, n)] = make_float2(vx.y, sg * vy.y);
-        pot[o0] = vp.x;
-        pot[o0 + n] = vp.y;
-        e += (double)(vp.x * rho[o0]) + (double)(vp.y * rho[o0 + n]);
+        if (pot_on) {
+            const float2 vp = buf[pos + (2 * F << logN)];
+            const size_t o0 = (size_t)row * n + j;
+            pot[o0] = vp.x;
+            pot[o0 + n] = vp.y;
+            e += (double)(vp.x * rho[o0]) + (double)(vp.y * rho[o0 + n]);
+        }
     }
     __shared__ double red[kBlock];
     red[threadIdx.x] = e;

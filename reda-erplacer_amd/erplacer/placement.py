@@ -129,6 +129,11 @@ class Placement:
     def set_keep_density(self, keep: bool):
         L.check(self._lib.erp_set_keep_density(self._h, C.c_int(int(keep))))
 
+    def set_compute_potential(self, on: bool):
+        """Potential map and energy every step (default on); off skips one of the
+        three inverse transforms -- Placement::step discards both (placement.rs:185)."""
+        L.check(self._lib.erp_set_compute_potential(self._h, C.c_int(1 if on else 0)))
+
     def set_store_gradients(self, store: bool):
         L.check(self._lib.erp_set_store_gradients(self._h, C.c_int(int(store))))
 

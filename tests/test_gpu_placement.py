@@ -253,3 +253,23 @@ def test_mfma_small_workload_end_state(gpu_lib):
     assert not a["nonfinite"]
     assert rel < 0.03
     assert abs(a["overflow"] - b["overflow"]) < 0.01
+
+
+@pytest.mark.gpu
+def test_skipping_the_potential_keeps_the_trajectory(gpu_lib):
+    """Placement::step discards the potential and energy (placement.rs:185):
+    skipping them changes nothing else, bit for bit; energy then reads 0."""
+    from erplacer import workloads as W
+    from erplacer.placement import Placement
+    d, g, iters, td = W.make("small")
+    with Placement(d, g, target_density=td) as A, Placement(d, g, target_density=td) as B:
+        B.set_compute_potential(False)
+        ia, ib = A.run(60), B.run(60)
+        assert ia["hpwl"] == ib["hpwl"] and ia["overflow"] == ib["overflow"]
+        assert ia["density_weight"] == ib["density_weight"] and ia["gamma"] == ib["gamma"]
+        xa, ya = A.positions()
+        xb, yb = B.positions()
+        assert np.array_equal(xa, xb) and np.array_equal(ya, yb)
+        assert ia["energy"] != 0.0 and ib["energy"] == 0.0
+        B.set_compute_potential(True)
+        assert B.run(1)["energy"] != 0.0
