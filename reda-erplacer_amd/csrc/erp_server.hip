@@ -202,6 +202,9 @@ void admitter_main(erp_server *s) {
         erp_step_info info{};
         if (!rc) {
             erp_set_store_gradients(j->ctx, 0);
+            // a job's result holds no energy: the potential map is not computed
+            // (the trajectory is unchanged, placement.rs:185 discards it too)
+            erp_set_compute_potential(j->ctx, 0);
             // the first chunk runs alone with eager launches (no per-job graph
             // capture; results are bit-identical to the graph-replayed loop)
             const int64_t first = std::min<int64_t>(s->cfg.check_every, j->max_iterations);
