@@ -162,6 +162,12 @@ int phase_scheduler(erp_ctx *ctx) {
     return rc;
 }
 
+// Workgroups of the WA launch on the graph's side branch (two per CU): its
+// loop over the plan leaves the spectral passes room on every CU (256: the
+// branch becomes the longer one; 1024: slightly slower; capping the
+// pin-gradient sum as well is slower).
+constexpr unsigned kSideBranchBlocks = 512;
+
 // One iteration (iteration >= 1) as a fused kernel sequence: what the graphs hold.
 // fork: (graph capture only) after the scatter, the WA kernel and the per-cell
 // sum of its pin gradients (k_pin2cell) run on a second branch beside the
@@ -175,7 +181,7 @@ int launch_fused_iteration(erp_ctx *ctx, bool fork = false) {
         if ((rc = run_scatter(ctx))) return rc;
         ERP_HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
         ERP_HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-        if ((rc = launch_wa(djob(ctx), ctx->dims, 1, ctx->side))) return rc;
+        if ((rc = launch_wa(djob(ctx), ctx->dims, 1, ctx->side, nullptr, kSideBranchBlocks))) return rc;
         if ((rc = launch_pin2cell(djob(ctx), ctx->dims, 1, ctx->side))) return rc;
         ERP_HIP_TRY(hipEventRecord(ctx->ev_join, ctx->side));
         if ((rc = run_spectral(ctx))) return rc;

@@ -158,8 +158,9 @@ struct BatchDims {
     void add(const JobDesc &j);         // widen to include job j
 };
 
+// max_blocks > 0 caps the grid (the kernels loop over the plan's workgroups)
 int launch_wa(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
-              KernelTimer *t = nullptr);
+              KernelTimer *t = nullptr, unsigned max_blocks = 0);
 int launch_sort(const JobDesc *d, const BatchDims &bd, int nj, bool owner_items, hipStream_t s,
                 KernelTimer *t = nullptr);   // owner_items: also build the owner-scatter work list
 // copy_perm: the sorted cell ids (aux.w) are copied back into perm, and the

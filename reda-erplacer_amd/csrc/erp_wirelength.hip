@@ -237,30 +237,37 @@ struct WaPlan {
     unsigned begin[kWaBuckets + 2];
 };
 
+// A grid smaller than the plan loops over its workgroups (grid stride): the
+// graph branch that runs beside the spectral passes caps its grid so that it
+// does not crowd their workgroups out of the CUs.
 __global__ void __launch_bounds__(kBlock) k_wa_narrow(const JobDesc *__restrict__ jd, WaPlan plan) {
     const WaNets &w = jd[blockIdx.y].wa;
-    int sg = 0;
-    while (sg + 1 < plan.nseg && blockIdx.x >= plan.begin[sg + 1]) ++sg;
-    const int64_t blk = (int64_t)blockIdx.x - plan.begin[sg];
-    switch (plan.bucket[sg]) {
-    case -1: wa_2pin(w, blk); break;
-    case 0: wa_group<1, 4>(w, 0, blk); break;
-    case 1: wa_group<2, 4>(w, 1, blk); break;
-    case 2: wa_group<4, 4>(w, 2, blk); break;
-    case 3: wa_group<8, 4>(w, 3, blk); break;
-    case 4: wa_group<16, 4>(w, 4, blk); break;
-    case 5: wa_group<32, 4>(w, 5, blk); break;
-    default: wa_group<64, 4>(w, 6, blk); break;
+    for (unsigned vb = blockIdx.x; vb < plan.begin[plan.nseg]; vb += gridDim.x) {
+        int sg = 0;
+        while (sg + 1 < plan.nseg && vb >= plan.begin[sg + 1]) ++sg;
+        const int64_t blk = (int64_t)vb - plan.begin[sg];
+        switch (plan.bucket[sg]) {
+        case -1: wa_2pin(w, blk); break;
+        case 0: wa_group<1, 4>(w, 0, blk); break;
+        case 1: wa_group<2, 4>(w, 1, blk); break;
+        case 2: wa_group<4, 4>(w, 2, blk); break;
+        case 3: wa_group<8, 4>(w, 3, blk); break;
+        case 4: wa_group<16, 4>(w, 4, blk); break;
+        case 5: wa_group<32, 4>(w, 5, blk); break;
+        default: wa_group<64, 4>(w, 6, blk); break;
+        }
     }
 }
 
 __global__ void __launch_bounds__(kBlock) k_wa_wide(const JobDesc *__restrict__ jd, WaPlan plan) {
     const WaNets &w = jd[blockIdx.y].wa;
-    int sg = 0;
-    while (sg + 1 < plan.nseg && blockIdx.x >= plan.begin[sg + 1]) ++sg;
-    const int64_t blk = (int64_t)blockIdx.x - plan.begin[sg];
-    if (plan.bucket[sg] == 7) wa_group<64, 8>(w, 7, blk);
-    else wa_large(w, 8, blk);
+    for (unsigned vb = blockIdx.x; vb < plan.begin[plan.nseg]; vb += gridDim.x) {
+        int sg = 0;
+        while (sg + 1 < plan.nseg && vb >= plan.begin[sg + 1]) ++sg;
+        const int64_t blk = (int64_t)vb - plan.begin[sg];
+        if (plan.bucket[sg] == 7) wa_group<64, 8>(w, 7, blk);
+        else wa_large(w, 8, blk);
+    }
 }
 
 // Exact HPWL: pin coordinates rounded to integer DBU (nearest-even).
@@ -323,7 +330,8 @@ int wa_partials(WaNets &w) {
     return total;
 }
 
-int launch_wa(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s, KernelTimer *t) {
+int launch_wa(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s, KernelTimer *t,
+              unsigned max_blocks) {
     // heavier workgroups first: high-degree buckets, then the small ones
     WaPlan narrow{}, wide{};
     unsigned total = 0;
@@ -337,7 +345,8 @@ int launch_wa(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s, Kern
     narrow.begin[narrow.nseg] = total;
     if (total) {
         if (t) t->begin("wa_narrow");
-        hipLaunchKernelGGL(k_wa_narrow, dim3(total, nj), dim3(kBlock), 0, s, d, narrow);
+        const unsigned grid = max_blocks ? std::min(total, max_blocks) : total;
+        hipLaunchKernelGGL(k_wa_narrow, dim3(grid, nj), dim3(kBlock), 0, s, d, narrow);
         if (t) t->end();
         ERP_HIP_TRY(hipGetLastError());
     }
@@ -352,7 +361,8 @@ int launch_wa(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s, Kern
     wide.begin[wide.nseg] = total;
     if (total) {
         if (t) t->begin("wa_wide");
-        hipLaunchKernelGGL(k_wa_wide, dim3(total, nj), dim3(kBlock), 0, s, d, wide);
+        const unsigned grid = max_blocks ? std::min(total, max_blocks) : total;
+        hipLaunchKernelGGL(k_wa_wide, dim3(grid, nj), dim3(kBlock), 0, s, d, wide);
         if (t) t->end();
         ERP_HIP_TRY(hipGetLastError());
     }
