@@ -93,3 +93,42 @@ def test_estimate_covers_allocation(gpu_lib):
         with Placement(d, g, target_density=1.0) as P:
             real = P.device_bytes()
         assert real <= est < 2 * real + (4 << 20), (real, est)
+
+
+def test_arrivals_while_running(gpu_lib):
+    """Continuous batching: jobs submitted while earlier ones are on the GPU are
+    admitted beside them (their [t_admit, t_done] intervals overlap), a design
+    that fails validation at admission fails alone, and every other result is
+    bit-identical to the job placed alone."""
+    import dataclasses
+    from erplacer.server import Server
+    jobs = _designs()
+    # first wave: one short job and two long ones (stop 0: they run to max_it)
+    first = [(0, 60, 0.25), (6, 1200, 0.0), (4, 1200, 0.0)]
+    second = [(1, 237, 0.25), (2, 237, 0.25), (3, 237, 0.25), (5, 237, 0.25)]
+    alone = {k: _alone(jobs[k][0], jobs[k][1], it, stop) for k, it, stop in first + second}
+    d_bad = jobs[1][0]
+    x_bad = d_bad.x.copy()
+    x_bad[d_bad.num_movable] = -5.0          # a fixed macro at a negative coordinate
+    d_bad = dataclasses.replace(d_bad, x=x_bad)
+    with Server(max_active=16, check_every=CHECK) as srv:
+        ids = {k: srv.submit(jobs[k][0], jobs[k][1], it, stop, target_density=1.0)
+               for k, it, stop in first}
+        assert srv.wait(ids[0])["status"] == 0          # the long jobs are still running
+        bad = srv.submit(d_bad, jobs[1][1], 100, 0.25, target_density=1.0)
+        for k, it, stop in second:
+            ids[k] = srv.submit(jobs[k][0], jobs[k][1], it, stop, target_density=1.0)
+        res = {k: srv.wait(i) for k, i in ids.items()}
+        rbad = srv.wait(bad)
+        for k, (info, x, y, dbu) in alone.items():
+            r = res[k]
+            assert r["status"] == 0, (k, r)
+            assert r["iterations"] == info["iteration"], (k, r, info)
+            assert r["hpwl"] == info["hpwl"] and r["overflow"] == info["overflow"]
+            assert r["hpwl_dbu"] == dbu
+            sx, sy = srv.positions(ids[k])
+            assert np.array_equal(sx, x) and np.array_equal(sy, y), k
+    assert rbad["status"] != 0
+    # a second-wave job was admitted before a long first-wave job finished
+    long_done = min(res[6]["t_done"], res[4]["t_done"])
+    assert any(res[k]["t_admit"] < long_done for k, _, _ in second)
