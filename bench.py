@@ -25,6 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "reda-erplacer_amd"))
 
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = 2500.0     # dense bf16 (no sparsity)
 METRIC = "GP iters/s @1M cells & placements/min + p50/p99 latency @ HPWL within 0.5%"
 
 
@@ -70,6 +71,18 @@ def kernel_bytes(name, d, grid, cells_pins):
         "update": C * 56,
     }
     return table.get(name, 0)
+
+
+def kernel_flops(name, grid, potential):
+    """MFMA flops one launch of the split-BF16 spectral form executes: every
+    pass is two half-size products (even / odd frequencies, K = L/2) of three
+    bf16 terms each, 3 m n^2 for a row pass and 3 m^2 n for a column pass, times
+    the transforms the launch carries (the inverses: 3, or 2 without the
+    potential map).  The useful FP32-equivalent work is a third of it."""
+    m = n = grid
+    nz = 3 if potential else 2
+    return {"mfma_fwd_row": 3 * m * n * n, "mfma_fwd_col": 3 * m * m * n,
+            "mfma_inv_col": nz * 3 * m * m * n, "mfma_inv_row": nz * 3 * m * n * n}.get(name, 0)
 
 
 # timer name -> kernel function (without the k_ prefix) in the rocprofv3 summaries
@@ -296,9 +309,17 @@ def main():
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(args.workload, dom) if args.spectral == "fft" else None,
             "bytes_per_launch": nbytes, "avg_launch_ms": round(dom_ms, 5)}
-    kernels = {k: {"avg_ms": round(v, 5),
-                   "gbs": round(kernel_bytes(k, d, grid, pins_cells) / (v * 1e-3) / 1e9, 1) if v > 0 else None}
-               for k, v in per_iter.items()}
+    kernels = {}
+    for k, v in per_iter.items():
+        e = {"avg_ms": round(v, 5)}
+        nb, nf = kernel_bytes(k, d, grid, pins_cells), kernel_flops(k, grid, not args.skip_potential)
+        if nb > 0 and v > 0:
+            gbs = nb / (v * 1e-3) / 1e9
+            e.update(gbs=round(gbs, 1), hbm_frac=round(gbs / HBM_PEAK_GBS, 4))
+        if nf > 0 and v > 0:
+            tf = nf / (v * 1e-3) / 1e12
+            e.update(mfma_tflops=round(tf, 1), mfma_frac=round(tf / MFMA_PEAK_TFLOPS, 4))
+        kernels[k] = e
     print(f"[bench] per-kernel avg ms: {kernels}", file=sys.stderr, flush=True)
 
     cpu = None
