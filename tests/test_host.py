@@ -98,6 +98,18 @@ def test_library_exports_every_declared_symbol():
     assert abs(cfg.density_weight0 - 8e-5) < 1e-20
 
 
+def test_rust_binding_declares_every_entry_point():
+    """The Rust extern blocks a maintainer copies declare every C-ABI function."""
+    import os
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    with open(os.path.join(root, "INTEGRATION.md")) as f:
+        text = f.read()
+    bound = set(re.findall(r"pub fn (erp_[a-z0-9_]+)\s*\(", text))
+    missing = [nm for nm in L.declared_functions() if nm not in bound]
+    assert not missing, missing
+
+
 def test_invalid_design_rejected_before_device_use():
     """Argument checks run before any HIP call (the reference panics instead)."""
     lib = L.load()
