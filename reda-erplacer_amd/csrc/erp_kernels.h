@@ -141,6 +141,8 @@ int launch_setup_cells(const float *w, const float *h, const float *npins, int64
 // blockIdx.y (or blockIdx.x for one-workgroup kernels), so jobs that share a bin
 // grid run as one launch per kernel; a single placement is a batch of one.
 struct JobDesc {
+    // 0: an empty slot of a serving batch (every batched kernel returns at once)
+    int32_t active = 1;
     WaNets wa;
     SortBufs sb;
     PotentialBufs pb;

@@ -119,6 +119,7 @@ __device__ __forceinline__ float2 combine(const CellBufs &c, int64_t i, float2 g
 }
 
 #define ERP_CELL_JOB()                                                              \
+    if (!jd[blockIdx.y].active) return; /* an empty slot of a batch */             \
     const CellBufs &c = jd[blockIdx.y].cb;                                         \
     const int64_t nblk = (c.C + kBlock - 1) / kBlock;                              \
     if ((int64_t)blockIdx.x >= nblk) return; /* uniform per workgroup */           \
@@ -242,6 +243,7 @@ __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
 }
 
 __global__ void __launch_bounds__(kScalarThreads) k_scalar(const JobDesc *__restrict__ jd, int ops) {
+    if (!jd[blockIdx.x].active) return;   // an empty slot of a batch
     const ScalarArgs &a = jd[blockIdx.x].sa;
     constexpr int kWaves = kScalarThreads / 64;
     __shared__ double redd[4][kWaves];
@@ -357,7 +359,7 @@ __global__ void __launch_bounds__(kScalarThreads) k_scalar(const JobDesc *__rest
 // (hpwl, overflow, nonfinite) of every job of a batch into one array.
 __global__ void k_gather_state(const JobDesc *__restrict__ jd, int nj, double *__restrict__ out) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nj) return;
+    if (j >= nj || !jd[j].active) return;
     const DevState *st = jd[j].sa.st;
     out[3 * j] = st->hpwl;
     out[3 * j + 1] = st->overflow;

@@ -75,12 +75,23 @@ struct Job {
     bool done = false;
 };
 
+// The jobs of one bin grid.  Its graph is captured for a fixed number of slots
+// (a power of two) and a launch geometry; a slot holds a job or is empty (its
+// table entry inactive: every kernel returns at once for it).  A job that
+// joins takes an empty slot and one that leaves empties its slot: only the
+// table is uploaded again.  The graph is captured again when no slot is free,
+// when a job needs a wider launch geometry, or when a quarter or less of the
+// slots are used.
 struct Group {
     int m = 0, n = 0;
-    std::vector<Job *> members;
-    bool dirty = true;
-    DevBuf table;                   // JobDesc[members]
-    BatchDims dims;
+    std::vector<Job *> slots;       // the graph's slots: a job or nullptr
+    std::vector<Job *> joining;     // jobs the next capture places into slots
+    std::vector<JobDesc> tab;       // host copy of the slot table
+    int members = 0;                // jobs in slots or joining
+    bool recapture = true;
+    bool table_dirty = false;
+    DevBuf table;                   // JobDesc[slots]
+    BatchDims dims;                 // launch geometry of the graph
     hipGraphExec_t graph = nullptr;
     DevBuf state_dev;               // (hpwl, overflow, nonfinite) per member, written by the graph
     double *state_host = nullptr;   // pinned copy, read after each round
@@ -247,29 +258,93 @@ void admitter_main(erp_server *s) {
     }
 }
 
+// b fits inside the launch geometry a
+bool covers(const BatchDims &a, const BatchDims &b) {
+    if (b.max_cells > a.max_cells || b.max_n2 > a.max_n2 || b.max_reach > a.max_reach) return false;
+    for (int k = 0; k < kWaBuckets; ++k)
+        if (b.max_bucket[k] > a.max_bucket[k]) return false;
+    return true;
+}
+
+void join(Group &g, Job *j) {
+    ++g.members;
+    BatchDims one;
+    one.add(j->ctx->job);
+    if (!g.recapture && g.graph && covers(g.dims, one)) {
+        auto it = std::find(g.slots.begin(), g.slots.end(), nullptr);
+        if (it != g.slots.end()) {
+            *it = j;
+            g.tab[(size_t)(it - g.slots.begin())] = j->ctx->job;
+            g.table_dirty = true;
+            return;
+        }
+    }
+    g.joining.push_back(j);
+    g.recapture = true;
+}
+
+// Removes j if it is a member; returns whether it was.
+bool leave(Group &g, Job *j) {
+    auto it = std::find(g.slots.begin(), g.slots.end(), j);
+    if (it != g.slots.end()) {
+        *it = nullptr;
+        g.tab[(size_t)(it - g.slots.begin())].active = 0;
+        g.table_dirty = true;
+    } else {
+        auto jt = std::find(g.joining.begin(), g.joining.end(), j);
+        if (jt == g.joining.end()) return false;
+        g.joining.erase(jt);
+    }
+    --g.members;
+    if (g.members > 0 && 4 * g.members <= (int)g.slots.size()) g.recapture = true;
+    return true;
+}
+
+int upload_table(erp_server *s, Group &g) {
+    g.table_dirty = false;
+    ERP_HIP_TRY(hipMemcpyAsync(g.table.p, g.tab.data(), sizeof(JobDesc) * g.tab.size(), hipMemcpyHostToDevice,
+                               s->stream));
+    ERP_HIP_TRY(hipStreamSynchronize(s->stream));
+    return ERP_OK;
+}
+
+// Captures the group's graph again: the members are packed into the first
+// slots of a power-of-two slot count, the launch geometry is theirs.
 int rebuild(erp_server *s, Group &g) {
     g.drop_graph();
-    g.dirty = false;
-    if (g.members.empty()) return ERP_OK;
-    std::vector<JobDesc> tab;
+    g.recapture = false;
+    std::vector<Job *> live;
+    for (Job *j : g.slots)
+        if (j) live.push_back(j);
+    live.insert(live.end(), g.joining.begin(), g.joining.end());
+    g.joining.clear();
+    g.slots.clear();
+    g.tab.clear();
+    if (live.empty()) return ERP_OK;
+    int nj = 1;
+    while (nj < (int)live.size()) nj <<= 1;
+    g.slots.assign((size_t)nj, nullptr);
+    g.tab.assign((size_t)nj, JobDesc{});
     g.dims = BatchDims();
-    for (Job *j : g.members) {
-        tab.push_back(j->ctx->job);
-        g.dims.add(j->ctx->job);
+    for (size_t k = 0; k < live.size(); ++k) {
+        g.slots[k] = live[k];
+        g.tab[k] = live[k]->ctx->job;
+        g.dims.add(live[k]->ctx->job);
     }
-    int rc = g.table.alloc(sizeof(JobDesc) * tab.size());
-    if (rc) return rc;
-    if (g.capacity < (int)tab.size()) {
-        const int cap = std::max<int>((int)tab.size(), s->cfg.max_active);
+    for (size_t k = live.size(); k < (size_t)nj; ++k) g.tab[k].active = 0;
+    int rc = ERP_OK;
+    if (g.table.bytes < sizeof(JobDesc) * (size_t)nj &&
+        (rc = g.table.alloc(sizeof(JobDesc) * (size_t)std::max(nj, s->cfg.max_active))))
+        return rc;
+    if (g.capacity < nj) {
+        const int cap = std::max(nj, s->cfg.max_active);
         if ((rc = g.state_dev.alloc(sizeof(double) * 3 * (size_t)cap))) return rc;
         if (g.state_host) (void)hipHostFree(g.state_host);
         g.state_host = nullptr;
         ERP_HIP_TRY(hipHostMalloc((void **)&g.state_host, sizeof(double) * 3 * (size_t)cap, hipHostMallocDefault));
         g.capacity = cap;
     }
-    ERP_HIP_TRY(hipMemcpyAsync(g.table.p, tab.data(), sizeof(JobDesc) * tab.size(), hipMemcpyHostToDevice, s->stream));
-    ERP_HIP_TRY(hipStreamSynchronize(s->stream));
-    const int nj = (int)tab.size();
+    if ((rc = upload_table(s, g))) return rc;
     ERP_HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < s->cfg.check_every && !rc; ++k)
         rc = launch_batched_iteration(g.table.as<JobDesc>(), g.dims, nj, s->stream);
@@ -281,8 +356,6 @@ int rebuild(erp_server *s, Group &g) {
     e = hipGraphInstantiate(&g.graph, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(ERP_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->max_batch = std::max<int64_t>(s->max_batch, nj);
     return ERP_OK;
 }
 
@@ -291,14 +364,14 @@ int rebuild(erp_server *s, Group &g) {
 int round(erp_server *s) {
     std::vector<Job *> running;
     std::vector<const double *> state;      // (hpwl, overflow, nonfinite) of running[k]
-    // rebuilds (re-uploads and re-captures) first, so that no batch is running
-    // while another group's buffers are reallocated
+    // captures and table uploads first, so that no batch is running while
+    // another group's buffers are reallocated or its table is written
     for (auto &kv : s->groups) {
         Group &g = kv.second;
-        if (g.dirty) {
-            int rc = rebuild(s, g);
-            if (rc) return rc;
-        }
+        int rc = ERP_OK;
+        if (g.recapture) rc = rebuild(s, g);
+        else if (g.table_dirty) rc = upload_table(s, g);
+        if (rc) return rc;
     }
     std::vector<hipStream_t> used;
     for (auto &kv : s->groups) {
@@ -309,12 +382,14 @@ int round(erp_server *s) {
         {
             std::lock_guard<std::mutex> lk(s->mu);
             ++s->batched_launches;
+            s->max_batch = std::max<int64_t>(s->max_batch, g.members);
         }
-        ERP_HIP_TRY(hipMemcpyAsync(g.state_host, g.state_dev.p, sizeof(double) * 3 * g.members.size(),
+        ERP_HIP_TRY(hipMemcpyAsync(g.state_host, g.state_dev.p, sizeof(double) * 3 * g.slots.size(),
                                    hipMemcpyDeviceToHost, g.stream));
         used.push_back(g.stream);
-        for (size_t k = 0; k < g.members.size(); ++k) {
-            running.push_back(g.members[k]);
+        for (size_t k = 0; k < g.slots.size(); ++k) {
+            if (!g.slots[k]) continue;
+            running.push_back(g.slots[k]);
             state.push_back(g.state_host + 3 * k);
         }
     }
@@ -343,14 +418,8 @@ int round(erp_server *s) {
         const double dt = retire(s, j, info);
         std::lock_guard<std::mutex> lk(s->mu);
         s->retire_s += dt;
-        for (auto &kv : s->groups) {
-            auto &mem = kv.second.members;
-            auto it = std::find(mem.begin(), mem.end(), j);
-            if (it != mem.end()) {
-                mem.erase(it);
-                kv.second.dirty = true;
-            }
-        }
+        for (auto &kv : s->groups)
+            if (leave(kv.second, j)) break;
         s->bytes_in_use -= j->bytes;
         s->unpolled.push_back(j->id);
         s->finished[j->id] = std::move(s->active[j->id]);
@@ -404,8 +473,7 @@ void worker_main(erp_server *s) {
                 Group &g = s->groups[{j->ctx->m, j->ctx->n}];
                 g.m = j->ctx->m;
                 g.n = j->ctx->n;
-                g.members.push_back(j);
-                g.dirty = true;
+                join(g, j);
                 s->active[j->id] = std::move(job);
             }
             s->ready.clear();
@@ -427,7 +495,7 @@ void worker_main(erp_server *s) {
             break;
         }
         for (auto it = s->groups.begin(); it != s->groups.end();) {
-            if (it->second.members.empty()) {
+            if (it->second.members == 0) {
                 it->second.drop_graph();
                 it = s->groups.erase(it);
             } else {
