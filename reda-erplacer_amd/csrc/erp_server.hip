@@ -1,20 +1,20 @@
 // Serving form: a persistent request loop that runs many placements at once on
 // one GPU with continuous batching.
 //
-// Requests (designs) are submitted from any thread and queue up.  An admission
-// thread takes queued jobs in arrival order while the job slots and the HBM
+// Requests (designs) are submitted from any thread and queue up.  Admission
+// threads take queued jobs in arrival order while the job slots and the HBM
 // budget allow (a job that does not fit waits, later smaller ones may pass it),
-// creates their contexts (state stays resident in device memory) and runs each
+// create their contexts (state stays resident in device memory) and run each
 // new job's first `check_every` iterations on the job's own stream, while the
-// batch worker keeps the running batches going on the server's stream; the
-// worker then adds the prepared job to the batch of its bin grid.  Each batch is
-// one hipGraph that
-// holds `check_every` iterations of every member, every kernel launched once for
-// all members (job = blockIdx.y).  After each round the worker reads every
-// job's scalars; a job whose overflow fell below its stop threshold, that
-// reached its iteration limit, or whose scalars became non-finite is retired:
-// its positions and exact HPWL are copied out, its device memory is freed and
-// queued jobs are admitted into the freed room, changing the batches.
+// batch worker keeps the running batches going on their streams; the worker
+// then adds the prepared job to the batch of its bin grid.  Each batch is one
+// hipGraph that holds `check_every` iterations of a fixed number of slots,
+// every kernel launched once for all slots (job = blockIdx.y; an empty slot
+// returns at once).  After each round the worker reads every job's scalars; a
+// job whose overflow fell below its stop threshold, that reached its iteration
+// limit, or whose scalars became non-finite is retired: its positions and exact
+// HPWL are copied out, its device memory is freed and queued jobs are admitted
+// into the freed room, taking the emptied slots.
 //
 // A job's kernels only read and write its own buffers and reduce in a fixed
 // order, so its result is bit-identical alone (erp_run_until with the same
