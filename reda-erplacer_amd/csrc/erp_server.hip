@@ -139,7 +139,7 @@ struct erp_server {
     std::thread worker;
     // context creation is mostly host work (net ordering, CSR, uploads): several
     // jobs are prepared at once
-    static constexpr int kAdmitters = 4;
+    static constexpr int kAdmitters = 8;
     std::vector<std::thread> admitters;
     // jobs holding a slot (admitted and not finished)
     size_t resident() const { return preparing.size() + ready.size() + active.size(); }
