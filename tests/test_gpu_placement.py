@@ -129,6 +129,28 @@ def test_graph_loop_matches_phase_loop(gpu_lib, small):
     assert A.info()["hpwl"] == B.info()["hpwl"]
 
 
+def test_graph_loop_matches_phase_loop_capped_wa(gpu_lib):
+    """At 350k cells the graph's side-branch WA launch (capped at 512 workgroups)
+    loops over a plan of more than twice as many workgroups; the loop is still
+    bit-identical to the phase kernels (uncapped WA, one workgroup per block)."""
+    import math
+    from erplacer import workloads as W
+    from erplacer.placement import Placement
+    d, g, _, td = W.make("large", scale=0.35)
+    assert math.ceil(d.n_2nets / 256) > 512        # the two-pin blocks alone exceed the cap
+    A = Placement(d, g, target_density=td)
+    B = Placement(d, g, target_density=td)
+    for _ in range(17):
+        A.step()
+    B.run(17)
+    xa, ya = A.positions()
+    xb, yb = B.positions()
+    assert np.array_equal(xa, xb) and np.array_equal(ya, yb)
+    assert A.info()["hpwl"] == B.info()["hpwl"]
+    A.close()
+    B.close()
+
+
 def test_deterministic(gpu_lib, small):
     from erplacer.placement import Placement
     d, g = small
