@@ -133,7 +133,6 @@ def test_graph_loop_matches_phase_loop_capped_wa(gpu_lib):
     """At 350k cells the graph's side-branch WA launch (capped at 512 workgroups)
     loops over a plan of more than twice as many workgroups; the loop is still
     bit-identical to the phase kernels (uncapped WA, one workgroup per block)."""
-    import math
     from erplacer import workloads as W
     from erplacer.placement import Placement
     d, g, _, td = W.make("large", scale=0.35)
