@@ -248,7 +248,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(jobs shard across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+              "(placement jobs shard across GPUs with no exchange: run one server per GPU)", file=sys.stderr)
         sys.exit(2)
 
     from erplacer import lib as L

@@ -14,7 +14,7 @@ from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
-# FP32 transforms vs FP64: relative L2 bound stated by the north star for the field.
+# FP32 transforms vs FP64: the relative L2 bound the field and gradient are held to.
 TOL_SPECTRAL = 1e-5
 
 
