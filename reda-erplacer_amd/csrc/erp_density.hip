@@ -60,7 +60,7 @@ __device__ __forceinline__ void add_q(int64_t *q, int64_t v) {
 // are the expensive resource here: they are coherent device-wide and far
 // slower than LDS atomics.
 __global__ void __launch_bounds__(kBlock) k_scatter(const JobDesc *__restrict__ jd, int copy_perm) {
-    if (!jd[blockIdx.y].active) return;   // an empty slot of a batch
+    if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
     __shared__ unsigned long long tile[kScatterTile];
     __shared__ int wbox[4][kBlock / 64];
     const SortBufs &sb = jd[blockIdx.y].sb;
@@ -197,7 +197,7 @@ __device__ __forceinline__ int tile_key(int tx, int ty) {
 
 __global__ void __launch_bounds__(kBlock) k_sort_keys(const JobDesc *__restrict__ jd, int shift,
                                                       int ntiles) {
-    if (!jd[blockIdx.y].active) return;   // an empty slot of a batch
+    if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
     __shared__ int lh[kSortTiles];
     const SortBufs &sb = jd[blockIdx.y].sb;
     const int64_t num_cells = sb.num_cells;
@@ -277,7 +277,7 @@ __device__ __forceinline__ int block_exclusive_scan(int sum, int *wsum, int &tot
 
 __global__ void __launch_bounds__(1024) k_sort_scan(const JobDesc *__restrict__ jd, int ntiles, int shift,
                                                    int owner_items) {
-    if (!jd[blockIdx.x].active) return;   // an empty slot of a batch
+    if (gridDim.x > 1 && !jd[blockIdx.x].active) return;   // an empty slot of a batch
     __shared__ int wsum[16];
     __shared__ int cnt[kSortTiles];
     const SortBufs &sb = jd[blockIdx.x].sb;
@@ -332,7 +332,7 @@ __global__ void __launch_bounds__(1024) k_sort_scan(const JobDesc *__restrict__ 
 }
 
 __global__ void __launch_bounds__(kBlock) k_sort_place(const JobDesc *__restrict__ jd, int ntiles) {
-    if (!jd[blockIdx.y].active) return;   // an empty slot of a batch
+    if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
     __shared__ int lh[kSortTiles];
     const SortBufs &sb = jd[blockIdx.y].sb;
     const int64_t num_cells = sb.num_cells;
@@ -409,7 +409,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_place(const JobDesc *__restrict
 // record back as the next sort's input.
 __global__ void __launch_bounds__(kBlock) k_scatter_owner(const JobDesc *__restrict__ jd, int shift,
                                                           int ntiles) {
-    if (!jd[blockIdx.y].active) return;   // an empty slot of a batch
+    if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
     __shared__ unsigned long long tile[32 * 32];
     __shared__ int src_start[9], src_cnt[9];
     const SortBufs &sb = jd[blockIdx.y].sb;

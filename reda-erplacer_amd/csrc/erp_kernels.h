@@ -141,7 +141,9 @@ int launch_setup_cells(const float *w, const float *h, const float *npins, int64
 // blockIdx.y (or blockIdx.x for one-workgroup kernels), so jobs that share a bin
 // grid run as one launch per kernel; a single placement is a batch of one.
 struct JobDesc {
-    // 0: an empty slot of a serving batch (every batched kernel returns at once)
+    // 0: an empty slot of a serving batch (every batched kernel returns at once).
+    // Launches over one job slot skip the check (gridDim 1 in the job
+    // dimension): a single context, or a batch of one, which is never empty.
     int32_t active = 1;
     WaNets wa;
     SortBufs sb;

@@ -119,7 +119,7 @@ __device__ __forceinline__ float2 combine(const CellBufs &c, int64_t i, float2 g
 }
 
 #define ERP_CELL_JOB()                                                              \
-    if (!jd[blockIdx.y].active) return; /* an empty slot of a batch */             \
+    if (gridDim.y > 1 && !jd[blockIdx.y].active) return; /* empty batch slot */    \
     const CellBufs &c = jd[blockIdx.y].cb;                                         \
     const int64_t nblk = (c.C + kBlock - 1) / kBlock;                              \
     if ((int64_t)blockIdx.x >= nblk) return; /* uniform per workgroup */           \
@@ -243,7 +243,7 @@ __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
 }
 
 __global__ void __launch_bounds__(kScalarThreads) k_scalar(const JobDesc *__restrict__ jd, int ops) {
-    if (!jd[blockIdx.x].active) return;   // an empty slot of a batch
+    if (gridDim.x > 1 && !jd[blockIdx.x].active) return;   // an empty slot of a batch
     const ScalarArgs &a = jd[blockIdx.x].sa;
     constexpr int kWaves = kScalarThreads / 64;
     __shared__ double redd[4][kWaves];

@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(kBlock) k_row_dct(const float *__restrict__ x,
 
 // pass 1 of the potential pipeline for job blockIdx.y
 __global__ void __launch_bounds__(kBlock) k_pass1(const JobDesc *__restrict__ jd, int logN, int F) {
-    if (!jd[blockIdx.y].active) return;   // an empty slot of a batch
+    if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
     const PotentialBufs &b = jd[blockIdx.y].pb;
     row_dct_body<true>(nullptr, b.q_mov, b.q_fix, b.rho, b.ovf_part, b.rowt, b.n, logN, F,
                        b.tn_fft, b.tn_dct, b.inv_scale, b.coef_q, b.inv_bin_area, !b.keep_q);
@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(kBlock) k_row_inv(const float *__restrict__ X,
 __global__ void __launch_bounds__(kBlock) k_col_spectral(const JobDesc *__restrict__ jd, int logM,
                                                          int G, float scale, float two_pi_over_m,
                                                          float two_pi_over_n) {
-    if (!jd[blockIdx.y].active) return;   // an empty slot of a batch
+    if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
     extern __shared__ float2 lds[];
     const PotentialBufs &pb = jd[blockIdx.y].pb;
     const float *__restrict__ T = pb.rowt;
@@ -478,7 +478,7 @@ __global__ void __launch_bounds__(kBlock) k_col_spectral(const JobDesc *__restri
 // field_x = IDCT_y(P1), field_y = IDXST_y(P2), potential = IDCT_y(P3);
 // energy partial = sum potential * density (epotential.rs:344-346).
 __global__ void __launch_bounds__(kBlock) k_row_final(const JobDesc *__restrict__ jd, int logN, int F) {
-    if (!jd[blockIdx.y].active) return;   // an empty slot of a batch
+    if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
     extern __shared__ float2 lds[];
     const PotentialBufs &pb = jd[blockIdx.y].pb;
     const float *__restrict__ P1 = pb.p1;

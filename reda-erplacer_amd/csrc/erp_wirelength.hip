@@ -241,7 +241,7 @@ struct WaPlan {
 // graph branch that runs beside the spectral passes caps its grid so that it
 // does not crowd their workgroups out of the CUs.
 __global__ void __launch_bounds__(kBlock) k_wa_narrow(const JobDesc *__restrict__ jd, WaPlan plan) {
-    if (!jd[blockIdx.y].active) return;   // an empty slot of a batch
+    if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
     const WaNets &w = jd[blockIdx.y].wa;
     for (unsigned vb = blockIdx.x; vb < plan.begin[plan.nseg]; vb += gridDim.x) {
         int sg = 0;
@@ -261,7 +261,7 @@ __global__ void __launch_bounds__(kBlock) k_wa_narrow(const JobDesc *__restrict_
 }
 
 __global__ void __launch_bounds__(kBlock) k_wa_wide(const JobDesc *__restrict__ jd, WaPlan plan) {
-    if (!jd[blockIdx.y].active) return;   // an empty slot of a batch
+    if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
     const WaNets &w = jd[blockIdx.y].wa;
     for (unsigned vb = blockIdx.x; vb < plan.begin[plan.nseg]; vb += gridDim.x) {
         int sg = 0;
