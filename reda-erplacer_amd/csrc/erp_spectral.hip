@@ -340,11 +340,16 @@ __global__ void __launch_bounds__(kBlock) k_row_inv(const float *__restrict__ X,
 }
 
 // ------------------------------------------------- fused column pass (pass 2)
+// Threads per workgroup of the column pass on grids of 1024 rows and more (at
+// 1024^2 the loop runs 4106-4116 it/s with 256, 4135-4149 with 512 and
+// 3928-3949 with 768); smaller grids keep kBlock.
+constexpr int kColThreadsLarge = 512;
+
 // Column DCT-II of the row-transformed map, a_uv = 4/(MN) * sums; the three
 // spectral products (frequencies.rs:4-72 evaluated in f32 on the fly, in the
 // reference's operation order); three column inverses:
 //   P1 = IDXST_x(a Wx), P2 = IDCT_x(a Wy), P3 = IDCT_x(a inv).
-__global__ void __launch_bounds__(kBlock) k_col_spectral(const JobDesc *__restrict__ jd, int logM,
+__global__ void __launch_bounds__(kColThreadsLarge) k_col_spectral(const JobDesc *__restrict__ jd, int logM,
                                                          int G, float scale, float two_pi_over_m,
                                                          float two_pi_over_n) {
     if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
@@ -678,7 +683,8 @@ int launch_potential_fft(const JobDesc *d, const BatchDims &bd, int nj, hipStrea
     size_t l2 = sizeof(float2) * ((size_t)m / 2 + (size_t)m + (size_t)3 * G * m);
     if ((rc = set_lds_limit((const void *)k_col_spectral, l2))) return rc;
     if (t) t->begin("pass2_col_spectral");
-    hipLaunchKernelGGL(k_col_spectral, dim3(n / (2 * G), nj), dim3(kBlock), l2, s, d, logM, G,
+    const unsigned col_threads = m >= 1024 ? kColThreadsLarge : kBlock;
+    hipLaunchKernelGGL(k_col_spectral, dim3(n / (2 * G), nj), dim3(col_threads), l2, s, d, logM, G,
                        (float)(4.0 / ((double)m * (double)n)), two_pi_over_m, two_pi_over_n);
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
