@@ -59,10 +59,10 @@ __device__ __forceinline__ void add_q(int64_t *q, int64_t v) {
 // adds its contributions with global atomics directly.  Global 64-bit atomics
 // are the expensive resource here: they are coherent device-wide and far
 // slower than LDS atomics.
-__global__ void __launch_bounds__(kBlock) k_scatter(const JobDesc *__restrict__ jd, int copy_perm) {
+__global__ void __launch_bounds__(kScatterThreads) k_scatter(const JobDesc *__restrict__ jd, int copy_perm) {
     if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
     __shared__ unsigned long long tile[kScatterTile];
-    __shared__ int wbox[4][kBlock / 64];
+    __shared__ int wbox[4][kScatterThreads / 64];
     const SortBufs &sb = jd[blockIdx.y].sb;
     const int64_t num_cells = sb.num_cells;
     const int64_t base = (int64_t)blockIdx.x * kScatterCells;
@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const JobDesc *__restrict__ 
     int lx = INT_MAX, hx = INT_MIN, ly = INT_MAX, hy = INT_MIN;
 #pragma unroll
     for (int j = 0; j < kScatterPer; ++j) {
-        const int64_t i = base + threadIdx.x + (int64_t)j * kBlock;
+        const int64_t i = base + threadIdx.x + (int64_t)j * kScatterThreads;
         bxl[j] = bxh[j] = byl[j] = byh[j] = 0;
         xi[j] = yi[j] = sx[j] = sy[j] = r[j] = 0.f;
         if (i < num_cells) {
@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const JobDesc *__restrict__ 
     __syncthreads();
     lx = wbox[0][0]; hx = wbox[1][0]; ly = wbox[2][0]; hy = wbox[3][0];
 #pragma unroll
-    for (int w = 1; w < kBlock / 64; ++w) {
+    for (int w = 1; w < kScatterThreads / 64; ++w) {
         lx = min(lx, wbox[0][w]); hx = max(hx, wbox[1][w]);
         ly = min(ly, wbox[2][w]); hy = max(hy, wbox[3][w]);
     }
@@ -133,7 +133,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const JobDesc *__restrict__ 
         ty0 = (ly + hy - wy) / 2;
     }
     const int area = wx * wy;
-    for (int t = threadIdx.x; t < area; t += kBlock) tile[t] = 0ull;
+    for (int t = threadIdx.x; t < area; t += kScatterThreads) tile[t] = 0ull;
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kScatterPer; ++j) {
@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const JobDesc *__restrict__ 
         }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < area; t += kBlock) {
+    for (int t = threadIdx.x; t < area; t += kScatterThreads) {
         const unsigned long long v = tile[t];
         if (v != 0ull) add_q(q + (size_t)(tx0 + t / wy) * n + (ty0 + t % wy), (int64_t)v);
     }
@@ -195,7 +195,7 @@ __device__ __forceinline__ int tile_key(int tx, int ty) {
     return (int)((spread_bits((unsigned)tx) << 1) | spread_bits((unsigned)ty));
 }
 
-__global__ void __launch_bounds__(kBlock) k_sort_keys(const JobDesc *__restrict__ jd, int shift,
+__global__ void __launch_bounds__(kScatterThreads) k_sort_keys(const JobDesc *__restrict__ jd, int shift,
                                                       int ntiles) {
     if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
     __shared__ int lh[kSortTiles];
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_keys(const JobDesc *__restrict_
     int32_t *__restrict__ hist = sb.hist;
     const int m = sb.m, n = sb.n;
     const float inv_bw = 1.f / sb.bw, inv_bh = 1.f / sb.bh;
-    for (int t = threadIdx.x; t < ntiles; t += kBlock) lh[t] = 0;
+    for (int t = threadIdx.x; t < ntiles; t += kScatterThreads) lh[t] = 0;
     __syncthreads();
     const int64_t base = xcd_block(blockIdx.x, nblk) * kScatterCells;
     // all loads of the thread's cells first (they are then in flight together)
@@ -220,7 +220,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_keys(const JobDesc *__restrict_
     float2 pv[kScatterPer];
 #pragma unroll
     for (int j = 0; j < kScatterPer; ++j) {
-        const int64_t i = base + threadIdx.x + (int64_t)j * kBlock;
+        const int64_t i = base + threadIdx.x + (int64_t)j * kScatterThreads;
         cid[j] = i < num_cells ? perm[i] : -1;
         stv[j] = i < num_cells ? sstat[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_keys(const JobDesc *__restrict_
     for (int j = 0; j < kScatterPer; ++j) pv[j] = cid[j] >= 0 ? pos[cid[j]] : make_float2(0.f, 0.f);
 #pragma unroll
     for (int j = 0; j < kScatterPer; ++j) {
-        const int64_t i = base + threadIdx.x + (int64_t)j * kBlock;
+        const int64_t i = base + threadIdx.x + (int64_t)j * kScatterThreads;
         const bool valid = i < num_cells;
         int k = -1;
         if (valid) {
@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_keys(const JobDesc *__restrict_
         }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < ntiles; t += kBlock)
+    for (int t = threadIdx.x; t < ntiles; t += kScatterThreads)
         if (lh[t]) atomicAdd(&hist[t], lh[t]);
 }
 
@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(1024) k_sort_scan(const JobDesc *__restrict__ 
     if (threadIdx.x == 0) sb.items[ntiles] = total;
 }
 
-__global__ void __launch_bounds__(kBlock) k_sort_place(const JobDesc *__restrict__ jd, int ntiles) {
+__global__ void __launch_bounds__(kScatterThreads) k_sort_place(const JobDesc *__restrict__ jd, int ntiles) {
     if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
     __shared__ int lh[kSortTiles];
     const SortBufs &sb = jd[blockIdx.y].sb;
@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_place(const JobDesc *__restrict
 
     float4 *__restrict__ rec = sb.rec;
     float4 *__restrict__ aux = sb.aux;
-    for (int t = threadIdx.x; t < ntiles; t += kBlock) lh[t] = 0;
+    for (int t = threadIdx.x; t < ntiles; t += kScatterThreads) lh[t] = 0;
     __syncthreads();
     const int64_t base = xcd_block(blockIdx.x, nblk) * kScatterCells;
     int key[kScatterPer], rank[kScatterPer];
@@ -358,7 +358,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_place(const JobDesc *__restrict
     // the thread's records are loaded up front, all in flight together
 #pragma unroll
     for (int j = 0; j < kScatterPer; ++j) {
-        const int64_t i = base + threadIdx.x + (int64_t)j * kBlock;
+        const int64_t i = base + threadIdx.x + (int64_t)j * kScatterThreads;
         const bool valid = i < num_cells;
         key[j] = valid ? keys[i] : -1;
         cperm[j] = valid ? perm[i] : 0;
@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_place(const JobDesc *__restrict
     }
 #pragma unroll
     for (int j = 0; j < kScatterPer; ++j) {
-        const int64_t i = base + threadIdx.x + (int64_t)j * kBlock;
+        const int64_t i = base + threadIdx.x + (int64_t)j * kScatterThreads;
         const bool valid = i < num_cells;
         rank[j] = 0;
         // valid lanes are a prefix of the wavefront; if they share one tile the
@@ -384,7 +384,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_place(const JobDesc *__restrict
         }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < ntiles; t += kBlock)
+    for (int t = threadIdx.x; t < ntiles; t += kScatterThreads)
         if (lh[t]) lh[t] = atomicAdd(&cursor[t], lh[t]);   // count -> reserved start
     __syncthreads();
 #pragma unroll
@@ -566,7 +566,7 @@ inline unsigned grid_for(int64_t count) { return (unsigned)((count + kBlock - 1)
 int launch_scatter(const JobDesc *d, const BatchDims &bd, int nj, bool copy_perm, hipStream_t s) {
     if (bd.max_cells <= 0) return ERP_OK;
     const unsigned nb = (unsigned)((bd.max_cells + kScatterCells - 1) / kScatterCells);
-    hipLaunchKernelGGL(k_scatter, dim3(nb, nj), dim3(kBlock), 0, s, d, copy_perm ? 1 : 0);
+    hipLaunchKernelGGL(k_scatter, dim3(nb, nj), dim3(kScatterThreads), 0, s, d, copy_perm ? 1 : 0);
     ERP_HIP_TRY(hipGetLastError());
     return ERP_OK;
 }
@@ -602,7 +602,7 @@ int launch_sort(const JobDesc *d, const BatchDims &bd, int nj, bool owner_items,
     const SortGeom g = sort_geometry(bd.m, bd.n);
     const unsigned nb = (unsigned)((bd.max_cells + kScatterCells - 1) / kScatterCells);
     if (t) t->begin("sort_keys");
-    hipLaunchKernelGGL(k_sort_keys, dim3(nb, nj), dim3(kBlock), 0, s, d, g.shift, g.ntiles);
+    hipLaunchKernelGGL(k_sort_keys, dim3(nb, nj), dim3(kScatterThreads), 0, s, d, g.shift, g.ntiles);
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
     if (t) t->begin("sort_scan");
@@ -610,7 +610,7 @@ int launch_sort(const JobDesc *d, const BatchDims &bd, int nj, bool owner_items,
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
     if (t) t->begin("sort_place");
-    hipLaunchKernelGGL(k_sort_place, dim3(nb, nj), dim3(kBlock), 0, s, d, g.ntiles);
+    hipLaunchKernelGGL(k_sort_place, dim3(nb, nj), dim3(kScatterThreads), 0, s, d, g.ntiles);
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
     return ERP_OK;

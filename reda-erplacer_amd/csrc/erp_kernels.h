@@ -17,8 +17,12 @@ __device__ __forceinline__ int64_t xcd_block(int64_t w, int64_t nb) {
 }
 
 // ------------------------------------------------------------ density
-constexpr int kScatterPer = 4;                       // cells per thread
-constexpr int kScatterCells = kScatterPer * kBlock;   // cells per workgroup
+// The sort and cell-major scatter kernels: 1024 cells per workgroup as 512
+// threads x 2 cells (at 1M cells 4141-4148 it/s as 256 x 4, 4272-4279 as
+// 512 x 2, 4280-4283 as 1024 x 1: more waves per CU hide the gathers).
+constexpr int kScatterThreads = 512;
+constexpr int kScatterPer = 2;                                 // cells per thread
+constexpr int kScatterCells = kScatterPer * kScatterThreads;   // cells per workgroup
 constexpr int kScatterTile = 8192;                    // LDS int64 bins per workgroup
 constexpr int kScatterSide = 128;                     // tile side when the box is too big
 constexpr int kSortTiles = 4096;                      // max sort tiles (64 x 64 Morton)
