@@ -158,6 +158,15 @@ int erp_set_store_gradients(erp_ctx *ctx, int store);
  * inverse transforms with an unchanged trajectory; energy then reads 0 and the
  * potential map is not refreshed. */
 int erp_set_compute_potential(erp_ctx *ctx, int on);
+/* Movable-scatter kernel of the loop (compute_density_map / _fallback,
+ * density.rs:240-299; both give the same int64 map bit for bit).  AUTO picks the
+ * owner scatter (one workgroup per tile neighbourhood, no atomics on exclusive
+ * tiles) for sparse grids whose cells reach at most two tiles past their own,
+ * the cell-major scatter (LDS tiles over sorted cells, global atomics) otherwise.
+ * Forcing OWNER where a cell reaches further is ERP_ERR_UNSUPPORTED. */
+enum { ERP_SCATTER_AUTO = -1, ERP_SCATTER_CELL_MAJOR = 0, ERP_SCATTER_OWNER = 1 };
+int erp_set_scatter_kind(erp_ctx *ctx, int kind);
+int erp_get_scatter_kind(erp_ctx *ctx, int32_t *kind);
 /* Clamped sizes/offsets/ratios of PotentialComputation::new (epotential.rs:83-103),
  * length num_nodes; scalars = {bin_w, bin_h, target_density, fx_scale}. */
 int erp_get_setup(erp_ctx *ctx, float *cw, float *ch, float *ox, float *oy, float *ratio,

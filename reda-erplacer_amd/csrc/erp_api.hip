@@ -105,15 +105,20 @@ int phase_hpwl(erp_ctx *ctx, const JobDesc *d) {
     return launch_scalar(d, 1, SC_HPWL, ctx->stream);
 }
 
+// Clears the movable accumulators if a step in keep mode left them filled.
+int clear_kept_density(erp_ctx *ctx) {
+    if (!ctx->q_dirty) return ERP_OK;
+    int rc = launch_zero_i64(ctx->q_mov.as<int64_t>(), ctx->B, ctx->stream);
+    if (rc == ERP_OK) ctx->q_dirty = false;
+    return rc;
+}
+
 // compute_potential (epotential.rs:225-374) in two halves: the movable density
 // (sort + scatter), then the spectral solve (density map -> field maps).
 int run_scatter(erp_ctx *ctx) {
     KernelTimer *t = ctx->timing ? ctx->timer : nullptr;
     int rc;
-    if (ctx->q_dirty) {
-        if ((rc = launch_zero_i64(ctx->q_mov.as<int64_t>(), ctx->B, ctx->stream))) return rc;
-        ctx->q_dirty = false;
-    }
+    if ((rc = clear_kept_density(ctx))) return rc;
     if ((rc = launch_sort(djob(ctx), ctx->dims, 1, ctx->owner_scatter, ctx->stream, t))) return rc;
     if (t) t->begin("scatter");
     rc = ctx->owner_scatter ? launch_scatter_owner(djob(ctx), ctx->dims, 1, ctx->stream)
@@ -543,9 +548,11 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     w.hpwl_part = ctx->hpwl_part.as<double>();
 
     ctx->n_pot_part = potential_partials(ctx->m);
-    ctx->n_energy_part = ctx->spectral_mode == ERP_SPECTRAL_MFMA_BF16X3 ? mfma_energy_partials(ctx->m, ctx->n)
-                                                                      : ctx->n_pot_part;
-    TRY(ctx->ovf_part.alloc(sizeof(int64_t) * (size_t)ctx->n_pot_part));
+    const bool mfma = ctx->spectral_mode == ERP_SPECTRAL_MFMA_BF16X3;
+    ctx->n_energy_part = mfma ? mfma_energy_partials(ctx->m, ctx->n) : ctx->n_pot_part;
+    // the scalar kernel sums exactly the overflow partials the form's first pass writes
+    ctx->n_ovf_part = mfma ? mfma_overflow_partials(ctx->m) : ctx->n_pot_part;
+    TRY(ctx->ovf_part.alloc(sizeof(int64_t) * (size_t)std::max(ctx->n_pot_part, ctx->n_ovf_part)));
     TRY(ctx->energy_part.alloc(sizeof(double) * (size_t)ctx->n_energy_part));
     PotentialBufs &pb = ctx->job.pb;
     pb.q_fix = ctx->q_fix.as<int64_t>();
@@ -611,7 +618,7 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     sa.hpwl_part = ctx->hpwl_part.as<double>();
     sa.n_hpwl = ctx->n_hpwl_part;
     sa.ovf_part = ctx->ovf_part.as<int64_t>();
-    sa.n_ovf = ctx->n_pot_part;
+    sa.n_ovf = ctx->n_ovf_part;
     sa.energy_part = ctx->energy_part.as<double>();
     sa.n_energy = ctx->n_energy_part;
     sa.part_a = ctx->part_a.as<double>();
@@ -677,6 +684,8 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
         // (2M cells / 2048^2: 157 -> 88 us), cell-major above (1M / 1024^2:
         // 40 vs 64 us).
         ctx->owner_scatter = sb.reach <= 2 && (double)C < 0.75 * (double)ctx->B;
+        ctx->owner_auto = ctx->owner_scatter;
+        ctx->reach = sb.reach;
     }
     for (DevBuf *b : {&ctx->hpwl_part, &ctx->ovf_part, &ctx->energy_part, &ctx->part_a, &ctx->part_b, &ctx->hist})
         if (hipMemsetAsync(b->p, 0, b->bytes, s) != hipSuccess) { erp_ctx_destroy(ctx); return fail(ERP_ERR_HIP, "hipMemsetAsync failed"); }
@@ -849,6 +858,8 @@ int erp_run(erp_ctx *ctx, int64_t iterations, erp_step_info *info) {
         }
         if (!rc && ctx->timing) rc = ctx->timer->collect();
     } else if (iterations > 0) {
+        // graphs hold no clearing node: accumulators left by a kept step are cleared first
+        if ((rc = clear_kept_density(ctx))) return rc;
         if (!ctx->graph_many && (rc = capture(ctx, kGraphIters, &ctx->graph_many))) return rc;
         if (!ctx->graph_one && (rc = capture(ctx, 1, &ctx->graph_one))) return rc;
         int64_t k = 0;
@@ -953,7 +964,10 @@ int erp_set_keep_density(erp_ctx *ctx, int keep) {
     if (g.rc) return g.rc;
     ctx->keep_q = keep != 0;
     ctx->job.pb.keep_q = ctx->keep_q;   // read by pass 1 from the device descriptor
-    return upload_job(ctx);
+    int rc = upload_job(ctx);
+    if (rc) return rc;
+    // the accumulators kept by the last step must not seed the next scatter
+    return clear_kept_density(ctx);
 }
 
 int erp_set_store_gradients(erp_ctx *ctx, int store) {
@@ -977,6 +991,30 @@ int erp_set_compute_potential(erp_ctx *ctx, int on) {
     ctx->job.pb.potential = on ? 1 : 0;
     ctx->job.sa.n_energy = on ? ctx->n_energy_part : 0;   // energy reads 0 while skipped
     return upload_job(ctx);
+}
+
+int erp_set_scatter_kind(erp_ctx *ctx, int kind) {
+    if (!ctx) return fail(ERP_ERR_INVALID, "null context");
+    bool owner;
+    if (kind == ERP_SCATTER_AUTO) owner = ctx->owner_auto;
+    else if (kind == ERP_SCATTER_CELL_MAJOR) owner = false;
+    else if (kind == ERP_SCATTER_OWNER) owner = true;
+    else return fail(ERP_ERR_INVALID, "unknown scatter kind");
+    if (owner && ctx->reach > 2)
+        return fail(ERP_ERR_UNSUPPORTED, "the owner scatter needs cells that reach at most two tiles past their own");
+    if (owner == ctx->owner_scatter) return ERP_OK;
+    DeviceGuard g(ctx->device);
+    if (g.rc) return g.rc;
+    ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    drop_graphs(ctx);          // the graphs hold the scatter kernel they were captured with
+    ctx->owner_scatter = owner;
+    return ERP_OK;
+}
+
+int erp_get_scatter_kind(erp_ctx *ctx, int32_t *kind) {
+    if (!ctx || !kind) return fail(ERP_ERR_INVALID, "null argument");
+    *kind = ctx->owner_scatter ? ERP_SCATTER_OWNER : ERP_SCATTER_CELL_MAJOR;
+    return ERP_OK;
 }
 
 int erp_get_setup(erp_ctx *ctx, float *cw, float *ch, float *ox, float *oy, float *ratio,

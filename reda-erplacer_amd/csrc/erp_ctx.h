@@ -34,12 +34,14 @@ struct erp_ctx {
     DevBuf state, gamma_override, hpwl_dbu;
     DevBuf keys, perm, hist, cursor, tstart, items, rec, aux, sstat, sratio, sll;
     bool owner_scatter = true;     // every movable cell reaches at most 2 tiles past its own
+    bool owner_auto = true;        // owner_scatter as chosen at creation (ERP_SCATTER_AUTO)
+    int reach = 1;                 // tiles a movable cell reaches past its own
     SpecTables tm, tn;
     MfmaPlan *mfma = nullptr;
     JobDesc job{};                 // host copy of the job descriptor
     DevBuf djob, djob_gamma;       // device copies (the second reads an explicit gamma)
     BatchDims dims;                // launch geometry of this placement alone
-    int n_hpwl_part = 0, n_pot_part = 0, n_energy_part = 0, n_cell_part = 0;
+    int n_hpwl_part = 0, n_pot_part = 0, n_ovf_part = 0, n_energy_part = 0, n_cell_part = 0;
     hipGraphExec_t graph_many = nullptr, graph_one = nullptr;
     KernelTimer *timer = nullptr;
     bool timing = false;

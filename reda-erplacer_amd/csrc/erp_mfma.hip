@@ -449,6 +449,9 @@ bool mfma_supported(int m, int n) {
 
 int mfma_energy_partials(int m, int n) { return (m / kBM) * ((n / 2) / kBN); }
 
+// k_finalize runs one workgroup per four rows and writes one overflow partial each.
+int mfma_overflow_partials(int m) { return m / 4; }
+
 int mfma_plan_create(MfmaPlan **out, int m, int n, hipStream_t s) {
     if (!mfma_supported(m, n))
         return fail(ERP_ERR_UNSUPPORTED, "the MFMA spectral form needs at least 128 bins per axis");
@@ -556,7 +559,7 @@ int launch_potential_mfma(const PotentialBufs &b, MfmaPlan *p, hipStream_t s, Ke
     if (!p || p->m != b.m || p->n != b.n) return fail(ERP_ERR_INVALID, "MFMA plan does not match the grid");
     const int m = b.m, n = b.n;
     if (t) t->begin("mfma_finalize");
-    hipLaunchKernelGGL(k_finalize, dim3((unsigned)(m / 4)), dim3(1024), 0, s, b);
+    hipLaunchKernelGGL(k_finalize, dim3((unsigned)mfma_overflow_partials(m)), dim3(1024), 0, s, b);
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
     MfmaArgs c = base_args(p, m, n);

@@ -19,6 +19,7 @@ ERP_OK, ERP_ERR_INVALID, ERP_ERR_HIP, ERP_ERR_NO_DEVICE, ERP_ERR_UNSUPPORTED = 0
 SPECTRAL_FFT, SPECTRAL_MFMA_BF16X3 = 0, 1
 MAP_DENSITY, MAP_AUV, MAP_FIELD_X, MAP_FIELD_Y, MAP_POTENTIAL, MAP_INITIAL = range(6)
 GRAD_WIRELENGTH, GRAD_DENSITY, GRAD_COMBINED = range(3)
+SCATTER_AUTO, SCATTER_CELL_MAJOR, SCATTER_OWNER = -1, 0, 1
 
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)

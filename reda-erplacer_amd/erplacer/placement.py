@@ -134,6 +134,15 @@ class Placement:
         three inverse transforms -- Placement::step discards both (placement.rs:185)."""
         L.check(self._lib.erp_set_compute_potential(self._h, C.c_int(1 if on else 0)))
 
+    def set_scatter_kind(self, kind: int):
+        """Scatter kernel of the loop (lib.SCATTER_AUTO / _CELL_MAJOR / _OWNER)."""
+        L.check(self._lib.erp_set_scatter_kind(self._h, C.c_int(kind)))
+
+    def scatter_kind(self) -> int:
+        v = C.c_int32()
+        L.check(self._lib.erp_get_scatter_kind(self._h, C.byref(v)))
+        return v.value
+
     def set_store_gradients(self, store: bool):
         L.check(self._lib.erp_set_store_gradients(self._h, C.c_int(int(store))))
 

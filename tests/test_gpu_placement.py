@@ -187,27 +187,58 @@ def test_final_hpwl_dbu_exact(gpu_lib, small):
     assert P.final_hpwl_dbu() == tot
 
 
-def test_small_workload_end_state(gpu_lib):
-    """Whole 'small' workload (10k cells, 128x128, 300 iterations): final HPWL
-    and overflow against the FP64 oracle run.  The reference's conflated
-    momentum (nesterov.rs:98-100,148-158) makes trajectories chaotic: in the FP64
-    oracle alone, accumulating the density map in fixed point instead of float
-    (a change of ~1e-13 per step) moves the final HPWL of this workload by 1.2%,
-    so the bound here is 3% in HPWL and 0.01 in overflow."""
+def _band():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                           "small_band.json")) as f:
+        return json.load(f)
+
+
+def _end_state(spectral_mode, label):
+    """Whole 'small' workload (10k cells, 128x128, 300 iterations) against the
+    FP64 oracle, step by step and at the end.
+
+    Steps 0-2 agree to rounding.  From step 3 on the f32 coordinates (the
+    reference binary's precision) take over: the first two steps move cells by
+    a few f32 ulps, so the Barzilai-Borwein step size of step 3 is known to
+    ~1e-3 (tests/test_oracle_trajectory.py shows the f32 oracle parting from
+    the FP64 one at the same step, by 1.0e-2 over steps 3-100).  The end state
+    is then one draw from the distribution the FP64 oracle itself produces
+    under one-ulp perturbations (tests/golden/small_band.json: 32 runs, HPWL
+    standard deviation 0.54 %, range 2.1 %): the bound is three standard
+    deviations of that ensemble around its mean."""
     from erplacer import workloads as W
     from erplacer.placement import Placement
     d, g, iters, td = W.make("small")
-    P = Placement(d, g, target_density=td)
-    a = P.run(iters)
+    P = Placement(d, g, target_density=td, spectral_mode=spectral_mode)
     ref = O.Placer(d, g, target_density=td)
+    gt, rt = [], []
     for _ in range(iters):
+        a = P.run(1)
         b = ref.step()
-    rel = abs(a["hpwl"] / b["hpwl"] - 1)
-    print(f"final hpwl gpu {a['hpwl']:.6e} oracle {b['hpwl']:.6e} rel {rel:.4%}; "
-          f"overflow gpu {a['overflow']:.5f} oracle {b['overflow']:.5f}")
-    assert not a["nonfinite"]
-    assert rel < 0.03
-    assert abs(a["overflow"] - b["overflow"]) < 0.01
+        assert not a["nonfinite"]
+        gt.append((a["hpwl"], a["overflow"]))
+        rt.append((b["hpwl"], b["overflow"]))
+    gt, rt = np.array(gt), np.array(rt)
+    rel = np.abs(gt[:, 0] / rt[:, 0] - 1)
+    dov = np.abs(gt[:, 1] - rt[:, 1])
+    s = _band()["fp64"]
+    zh = (gt[-1, 0] - s["hpwl_mean"]) / s["hpwl_std"]
+    zo = (gt[-1, 1] - s["overflow_mean"]) / s["overflow_std"]
+    print(f"{label}: steps 0-2 hpwl rel {rel[:3].max():.1e}, steps 3-100 {rel[3:101].max():.1e}, "
+          f"overflow 0-100 {dov[:101].max():.1e}; final hpwl {gt[-1, 0]:.6e} "
+          f"(FP64 run {rt[-1, 0]:.6e}, rel {rel[-1]:.3%}; ensemble z {zh:+.2f}), overflow "
+          f"{gt[-1, 1]:.5f} (FP64 run {rt[-1, 1]:.5f}; ensemble z {zo:+.2f})")
+    assert rel[:3].max() < 1e-6
+    assert rel[3:101].max() < 1e-2 and dov[:101].max() < 1e-3
+    assert abs(zh) <= 3.0 and abs(zo) <= 3.0
+    P.close()
+
+
+def test_small_workload_end_state(gpu_lib):
+    from erplacer import lib as L
+    _end_state(L.SPECTRAL_FFT, "fft")
 
 
 def test_mfma_step0_maps(gpu_lib):
@@ -257,23 +288,8 @@ def test_mfma_graph_loop_matches_phase_loop(gpu_lib, small):
 
 
 def test_mfma_small_workload_end_state(gpu_lib):
-    """The whole 'small' workload with the MFMA spectral form against the FP64
-    oracle run (bounds as for the FFT form: the trajectory is chaotic)."""
     from erplacer import lib as L
-    from erplacer import workloads as W
-    from erplacer.placement import Placement
-    d, g, iters, td = W.make("small")
-    P = Placement(d, g, target_density=td, spectral_mode=L.SPECTRAL_MFMA_BF16X3)
-    a = P.run(iters)
-    ref = O.Placer(d, g, target_density=td)
-    for _ in range(iters):
-        b = ref.step()
-    rel = abs(a["hpwl"] / b["hpwl"] - 1)
-    print(f"mfma final hpwl gpu {a['hpwl']:.6e} oracle {b['hpwl']:.6e} rel {rel:.4%}; "
-          f"overflow gpu {a['overflow']:.5f} oracle {b['overflow']:.5f}")
-    assert not a["nonfinite"]
-    assert rel < 0.03
-    assert abs(a["overflow"] - b["overflow"]) < 0.01
+    _end_state(L.SPECTRAL_MFMA_BF16X3, "mfma")
 
 
 @pytest.mark.gpu

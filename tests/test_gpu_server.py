@@ -132,3 +132,33 @@ def test_arrivals_while_running(gpu_lib):
     # a second-wave job was admitted before a long first-wave job finished
     long_done = min(res[6]["t_done"], res[4]["t_done"])
     assert any(res[k]["t_admit"] < long_done for k, _, _ in second)
+
+
+def test_alone_equals_mix_large_jobs(gpu_lib):
+    """The serving stream's largest jobs: 150k cells on the grid the stream's
+    rule gives (256^2) and on 512^2 grids, beside smaller jobs; two jobs share
+    each of the 512^2 and 256^2 batches.  Every result equals its alone run."""
+    from erplacer import workloads as W
+    from erplacer.netlist import generate
+    from erplacer.server import Server
+    specs = [(150_000, 512, 61), (150_000, W.serving_grid(150_000), 62), (90_000, 512, 63),
+             (60_000, 256, 64), (20_000, 128, 65)]
+    assert W.serving_grid(150_000) == 256
+    jobs = [(generate(c, int(c * 1.05), 128, 20, 0.1, 0.75, s, name=f"big{s}"), g)
+            for c, g, s in specs]
+    max_it, stop = 400, 0.1
+    alone = [_alone(d, g, max_it, stop) for d, g in jobs]
+    with Server(max_active=8, check_every=CHECK) as srv:
+        ids = [srv.submit(d, g, max_it, stop, target_density=1.0) for d, g in jobs]
+        res = {i: srv.wait(i) for i in ids}
+        st = srv.stats()
+        for jid, (info, x, y, dbu) in zip(ids, alone):
+            r = res[jid]
+            assert r["status"] == 0
+            assert r["iterations"] == info["iteration"], (jid, r, info)
+            assert r["hpwl"] == info["hpwl"] and r["overflow"] == info["overflow"]
+            assert r["hpwl_dbu"] == dbu
+            sx, sy = srv.positions(jid)
+            assert np.array_equal(sx, x) and np.array_equal(sy, y), jid
+    assert st["max_batch"] >= 2
+    print("server stats", st)
