@@ -107,31 +107,79 @@ def measured_traffic(workload, kernel):
 SEQUENTIAL_SCATTER_ITERS = 200
 
 
-def cpu_baseline(d, grid, td, workload_iters, par_iters, threads):
-    """The oracle's f32 OpenMP path on the same workload: every iteration of the
+def cpu_baseline(d, grid, td, workload_iters, par_iters, threads, repeats=3):
+    """The oracle's f32 OpenMP path on the same workload, `repeats` runs from
+    iteration 0 (the median is reported): every iteration of the
     sequential-scatter phase is timed, then `par_iters` of the parallel phase;
-    the rate is the workload's iteration count over the time the two phases
-    would take at those per-iteration costs.  Returns (rate, seconds timed,
-    iterations timed)."""
+    a run's rate is the workload's iteration count over the time the two
+    phases would take at those per-iteration costs.  Returns (median rate,
+    all rates, seconds timed, iterations timed per run)."""
+    import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    P = O.Placer(d, grid, target_density=td, sfx="_f32", nthreads=threads)
     n_seq = min(SEQUENTIAL_SCATTER_ITERS, workload_iters)
     n_par = workload_iters - n_seq
-    t0 = time.perf_counter()
-    for _ in range(n_seq):
-        P.step()
-    t_seq = time.perf_counter() - t0
     k = min(par_iters, n_par)
-    t1 = time.perf_counter()
-    for _ in range(k):
-        P.step()
-    t_par = time.perf_counter() - t1
-    projected = t_seq + (n_par * t_par / k if k else 0.0)
-    return workload_iters / projected, t_seq + t_par, n_seq + k
+    rates, timed = [], 0.0
+    for _ in range(repeats):
+        P = O.Placer(d, grid, target_density=td, sfx="_f32", nthreads=threads)
+        t0 = time.perf_counter()
+        for _ in range(n_seq):
+            P.step()
+        t_seq = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        for _ in range(k):
+            P.step()
+        t_par = time.perf_counter() - t1
+        projected = t_seq + (n_par * t_par / k if k else 0.0)
+        rates.append(workload_iters / projected)
+        timed += t_seq + t_par
+        del P
+    return float(np.median(rates)), rates, timed, n_seq + k
 
 
-def serving_bench(n_requests, rates, check_every, max_active, max_iterations):
+def cpu_serving_baseline(reqs, check_every, max_iterations, threads, budget_s):
+    """Sequential plain solves on the CPU: the oracle's f32 OpenMP loop places the
+    serving stream's requests one after another (stop at overflow < 0.1 checked
+    every `check_every` iterations, as the GPU server does) until `budget_s`
+    seconds have passed; returns placements/min over the requests finished."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from erplacer import workloads as W
+    stop = W.SERVING["stop_overflow"]
+    td = W.SERVING["target_density"]
+    done, cells, t0 = 0, 0, time.perf_counter()
+    for _, d, g in reqs:
+        P = O.Placer(d, g, target_density=td, sfx="_f32", nthreads=threads)
+        it = 0
+        while it < max_iterations:
+            for _ in range(min(check_every, max_iterations - it)):
+                r = P.step()
+                it += 1
+            if r["overflow"] < stop:
+                break
+        del P
+        done += 1
+        cells += d.num_movable
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"placements_per_min": round(60 * done / dt, 2), "requests": done,
+            "cells": cells, "seconds": round(dt, 2), "cores": threads, "kind": "port",
+            "sample": f"the first {done} requests of the stream placed one after another "
+                      f"(f32 OpenMP restatement, context setup included)"}
+
+
+def serving_requests(n_requests, rate):
+    from erplacer import workloads as W
+    t0 = time.perf_counter()
+    reqs = W.serving_requests(n_requests, rate_per_s=rate)
+    print(f"[serving] generated {len(reqs)} requests in {time.perf_counter() - t0:.1f}s "
+          f"({sum(d.num_movable for _, d, _ in reqs)} cells)", file=sys.stderr, flush=True)
+    return reqs
+
+
+def serving_bench(reqs, rates, check_every, max_active, max_iterations):
     """Serving form: the request stream (log-uniform 2k-150k cells, grids 64-512,
     stop at overflow < 0.1) placed (1) one after another alone, (2) all submitted at
     once to the continuous-batching server, (3) with Poisson arrivals at each rate
@@ -143,11 +191,6 @@ def serving_bench(n_requests, rates, check_every, max_active, max_iterations):
     from erplacer.server import Server
     stop = W.SERVING["stop_overflow"]
     td = W.SERVING["target_density"]
-    t0 = time.perf_counter()
-    reqs = W.serving_requests(n_requests, rate_per_s=rates[0])
-    print(f"[serving] generated {len(reqs)} requests in {time.perf_counter() - t0:.1f}s "
-          f"({sum(d.num_movable for _, d, _ in reqs)} cells)", file=sys.stderr, flush=True)
-
     # (1) sequential plain solves (context creation included)
     seq_t, seq_res = [], []
     for _, d, g in reqs:
@@ -216,6 +259,40 @@ def serving_bench(n_requests, rates, check_every, max_active, max_iterations):
     }
 
 
+def end_state(workload, info, hpwl_dbu):
+    """The end state of the whole workload against the FP64 oracle's runs of the
+    same workload (tests/golden/<workload>_band.json, written by the oracle on
+    the CPU): the unperturbed FP64 run and the spread of its one-ulp
+    perturbations.  The metric's qualifier ('HPWL within 0.5 %') is reported
+    as measured, beside the band the oracle itself produces."""
+    path = os.path.join(ROOT, "tests", "golden", f"{workload}_band.json")
+    out = {"hpwl": info["hpwl"], "overflow": info["overflow"], "hpwl_dbu": hpwl_dbu,
+           "iteration": info["iteration"]}
+    if not os.path.exists(path):
+        out["fp64_reference"] = None
+        return out
+    with open(path) as f:
+        band = json.load(f)
+    ref = band["runs"][0]                       # unperturbed, float map (the faithful oracle)
+    s = band["fp64"]
+    rel = info["hpwl"] / ref["hpwl"] - 1
+    z = (info["hpwl"] - s["hpwl_mean"]) / s["hpwl_std"]
+    zo = (info["overflow"] - s["overflow_mean"]) / s["overflow_std"]
+    out.update({
+        "fp64_run_hpwl": ref["hpwl"], "fp64_run_overflow": ref["overflow"],
+        "hpwl_rel_to_fp64_run": round(rel, 6),
+        "overflow_diff_to_fp64_run": round(info["overflow"] - ref["overflow"], 6),
+        "within_0.5pct_and_0.005": bool(abs(rel) <= 0.005 and abs(info["overflow"] - ref["overflow"]) <= 0.005),
+        "fp64_band": {"runs": s["n"], "hpwl_mean": s["hpwl_mean"],
+                      "hpwl_std_rel": round(s["hpwl_std"] / s["hpwl_mean"], 6),
+                      "hpwl_range_rel": round(s["hpwl_max"] / s["hpwl_min"] - 1, 6),
+                      "overflow_mean": s["overflow_mean"], "overflow_std": s["overflow_std"]},
+        "hpwl_z_in_band": round(z, 3), "overflow_z_in_band": round(zo, 3),
+        "within_band_3sigma": bool(abs(z) <= 3 and abs(zo) <= 3),
+    })
+    return out
+
+
 def copy_calibration(nbytes, reps):
     """Device-to-device hipMemcpy (the runtime's blit kernel) of a known size,
     the reference dispatch for scaling FETCH_SIZE / WRITE_SIZE counters."""
@@ -245,18 +322,22 @@ def main():
                          "Placement::step discards (placement.rs:185); not the default")
     ap.add_argument("--timing-steps", type=int, default=0,
                     help="iterations re-run with per-kernel HIP events for the roofline "
-                         "(default: the same number as --steps)")
-    ap.add_argument("--cpu-iters", type=int, default=60,
+                         "(default: the workload's full iteration count, whatever --steps is)")
+    ap.add_argument("--cpu-iters", type=int, default=20,
                     help="timed CPU iterations of the parallel-scatter phase (the "
                          "sequential phase before it is timed whole)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-repeats", type=int, default=3,
+                    help="CPU baseline runs (the median is reported)")
+    ap.add_argument("--cpu-serving-seconds", type=float, default=15.0,
+                    help="time budget of the CPU sequential-solve leg of the serving form")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="only the per-kernel timing loop, the launches whose average the "
                          "roofline quotes (for rocprofv3 runs)")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run a plain 256 MiB device copy (torch) after the loop, the "
-                         "reference kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE scaling")
+                    help="also run a plain 256 MiB device-to-device hipMemcpy after the loop, "
+                         "the reference dispatch for rocprofv3 FETCH_SIZE / WRITE_SIZE scaling")
     ap.add_argument("--serving-requests", type=int, default=512,
                     help="requests of the serving measurement (0 skips it)")
     ap.add_argument("--serving-rate", default="20,60",
@@ -289,10 +370,11 @@ def main():
         P.run(args.warmup)
     P.reset()
     P.info()  # synchronise
+    window = args.timing_steps or iters_cfg
     if args.profile:
         P.reset()
         P.set_timing(True)
-        P.run(args.timing_steps or args.steps)
+        P.run(window)
         P.set_timing(False)
         t = P.timing()
         print(f"[bench] profile loop per-kernel avg ms: { {k: round(v, 5) for k, v in t.items()} }",
@@ -308,10 +390,24 @@ def main():
           f"hpwl {info['hpwl']:.6e} overflow {info['overflow']:.5f}", file=sys.stderr, flush=True)
     hpwl_dbu = P.final_hpwl_dbu()
 
-    # per-kernel device time with HIP events over a second timed region
+    # the whole workload (its own iteration count from iteration 0), for the
+    # end state and the rate over every iteration of it
+    if args.steps == iters_cfg:
+        full_ips, full_info, full_dbu = ips, info, hpwl_dbu
+    else:
+        P.reset()
+        P.info()
+        t0 = time.perf_counter()
+        full_info = P.run(iters_cfg)
+        full_ips = iters_cfg / (time.perf_counter() - t0)
+        full_dbu = P.final_hpwl_dbu()
+
+    # per-kernel device time with HIP events over a second timed region: the
+    # workload's full iteration count, so the averages cover the clustered start
+    # and the spread end alike (what a kernel trace of the whole run averages)
     P.reset()
     P.set_timing(True)
-    P.run(args.timing_steps or args.steps)
+    P.run(window)
     P.set_timing(False)
     times = P.timing()
     if not times:
@@ -328,7 +424,8 @@ def main():
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(args.workload, dom) if args.spectral == "fft" else None,
-            "bytes_per_launch": nbytes, "avg_launch_ms": round(dom_ms, 5)}
+            "bytes_per_launch": nbytes, "avg_launch_ms": round(dom_ms, 5),
+            "window_iterations": window}
     kernels = {}
     for k, v in per_iter.items():
         e = {"avg_ms": round(v, 5)}
@@ -342,21 +439,31 @@ def main():
         kernels[k] = e
     print(f"[bench] per-kernel avg ms: {kernels}", file=sys.stderr, flush=True)
 
+    rates = [float(r) for r in str(args.serving_rate).split(",") if r]
+    reqs = serving_requests(args.serving_requests, rates[0]) if args.serving_requests > 0 else None
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cps, cdt, cn = cpu_baseline(d, grid, td, iters_cfg, args.cpu_iters, threads)
+        cps, crates, cdt, cn = cpu_baseline(d, grid, td, iters_cfg, args.cpu_iters, threads,
+                                            args.cpu_repeats)
         cpu = {"value": round(cps, 4), "unit": "GP iterations/s", "cores": threads, "kind": "port",
-               "sample": f"first {cn} iterations of the same workload ({cdt:.1f}s; the "
-                         f"sequential-scatter phase whole, then {cn - min(SEQUENTIAL_SCATTER_ITERS, iters_cfg)} "
-                         f"of the parallel one, projected onto its {iters_cfg}), f32, OpenMP "
-                         f"restatement of the Rayon path"}
-        print(f"[bench] cpu baseline {cps:.4f} it/s on {threads} threads", file=sys.stderr, flush=True)
+               "runs": [round(r, 4) for r in crates],
+               "sample": f"median of {len(crates)} runs, each the first {cn} iterations of the "
+                         f"same workload ({cdt:.1f}s in all; the sequential-scatter phase whole, "
+                         f"then {cn - min(SEQUENTIAL_SCATTER_ITERS, iters_cfg)} of the parallel "
+                         f"one, projected onto its {iters_cfg}), f32, OpenMP restatement of the "
+                         f"Rayon path"}
+        print(f"[bench] cpu baseline {cps:.4f} it/s on {threads} threads (runs {crates})",
+              file=sys.stderr, flush=True)
+        if args.serving_requests > 0 and args.cpu_serving_seconds > 0:
+            cpu["serving"] = cpu_serving_baseline(reqs, args.serving_check_every,
+                                                  args.serving_max_iterations, threads,
+                                                  args.cpu_serving_seconds)
+            print(f"[bench] cpu serving baseline {cpu['serving']}", file=sys.stderr, flush=True)
 
     serving = None
-    if args.serving_requests > 0:
-        rates = [float(r) for r in str(args.serving_rate).split(",") if r]
-        serving = serving_bench(args.serving_requests, rates, args.serving_check_every,
+    if reqs:
+        serving = serving_bench(reqs, rates, args.serving_check_every,
                                 args.serving_max_active, args.serving_max_iterations)
         print(f"[serving] {serving}", file=sys.stderr, flush=True)
 
@@ -372,6 +479,8 @@ def main():
                    "parallelism": "single GPU"},
         "final": {"hpwl": info["hpwl"], "overflow": info["overflow"], "hpwl_dbu": hpwl_dbu,
                   "iteration": info["iteration"]},
+        "full_workload": {"iterations": iters_cfg, "it_per_s": round(full_ips, 2),
+                          "end_state": end_state(args.workload, full_info, full_dbu)},
         "roofline": roof, "kernels": kernels, "cpu_baseline": cpu, "serving": serving,
     }
     print(json.dumps(out), flush=True)
