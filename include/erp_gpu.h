@@ -252,6 +252,23 @@ int erp_idct2_fft2_forward(const float *x, float *out, int32_t m, int32_t n, int
 int erp_idct_idxst_forward(const float *x, float *out, int32_t m, int32_t n, int32_t mode);
 /* idxst_idct_forward (idxst_idct.rs:156-180): sine along x, cosine along y */
 int erp_idxst_idct_forward(const float *x, float *out, int32_t m, int32_t n, int32_t mode);
+/* Allocation-free form of the four transforms above, on device buffers.  It has
+ * no counterpart in the reference; it is for callers that cannot allocate
+ * device memory: code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  x and out are device float[m*n] of the current
+ * device (not aliased), which: 0 dct2_fft2_forward, 1 idct2_fft2_forward,
+ * 2 idct_idxst_forward, 3 idxst_idct_forward (FFT form, the same kernels and
+ * results as the host-buffer entry points), stream a hipStream_t of the current
+ * device (NULL: the default stream).  The call only enqueues kernels on that
+ * stream (out is valid once the stream reaches them) and makes no allocation,
+ * stream or event: its scratch is one fixed array per GPU of 16 MiB + 48 KiB
+ * (one 2048 x 2048 map and the tables of two lengths), so m and n are powers
+ * of two in [64, 2048] and larger sizes are ERP_ERR_UNSUPPORTED.  Calls that
+ * overlap on one GPU share that array and race; serialise them (one stream, or
+ * events).  The first call on a device looks the array's address up; make it
+ * outside graph capture. */
+int erp_spectral_transform_noalloc(const float *x, float *out, int32_t m, int32_t n,
+                                   int32_t which, void *stream);
 /* compute_density_map / _fallback (density.rs:240-299): movable scatter of
  * clamped cells (x, y are cell origins; ox, oy the clamp offsets) accumulated in
  * fixed point at `scale`, then + initial.  q_out (nullable) receives the

@@ -1132,13 +1132,7 @@ int leaf_spectral(const float *x, float *out, int32_t m, int32_t n, int32_t mode
         return rc;
     const float *px = dx.as<float>();
     float *po = dout.as<float>(), *ps = dscr.as<float>();
-    switch (which) {
-    case 0: rc = launch_dct2(px, po, ps, m, n, tm, tn, L.s); break;
-    case 1: rc = launch_inverse2(px, po, ps, m, n, INV_IDCT, INV_IDCT, tm, tn, L.s); break;
-    case 2: rc = launch_inverse2(px, po, ps, m, n, INV_IDCT, INV_IDXST, tm, tn, L.s); break;
-    default: rc = launch_inverse2(px, po, ps, m, n, INV_IDXST, INV_IDCT, tm, tn, L.s); break;
-    }
-    if (rc) return rc;
+    if ((rc = launch_spectral_leaf(px, po, ps, m, n, which, spec_view(tm), spec_view(tn), L.s))) return rc;
     ERP_HIP_TRY(hipMemcpyAsync(out, po, sizeof(float) * B, hipMemcpyDeviceToHost, L.s));
     ERP_HIP_TRY(hipStreamSynchronize(L.s));
     return ERP_OK;

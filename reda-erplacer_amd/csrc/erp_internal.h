@@ -130,19 +130,32 @@ struct SpecTables {
     DevBuf dct_tw;   // float2[L]:   e^{-i pi k / (2L)}
 };
 
+// Device pointers to the tables of one length: a SpecTables or memory the
+// caller provides (the allocation-free entry point's scratch).
+struct SpecView {
+    const float2 *fft_tw, *dct_tw;
+};
+inline SpecView spec_view(const SpecTables &t) { return {t.fft_tw.as<float2>(), t.dct_tw.as<float2>()}; }
+
 // ----------------------------------------------------------- launch helpers
 int make_spec_tables(SpecTables &t, int L, hipStream_t s);
+// fills fft_tw[L/2] and dct_tw[L] (no allocation)
+int launch_make_tables(float2 *fft_tw, float2 *dct_tw, int L, hipStream_t s);
 
 // Spectral operators (erp_spectral.hip).  All buffers are device float[m*n].
 // Kinds of one-dimensional inverse transforms:
 enum InvKind { INV_IDCT = 0, INV_IDXST = 1 };
 
 // a = dctn(x, 2) / (m n)     (scratch: float[m*n])
-int launch_dct2(const float *x, float *out, float *scratch, int m, int n,
-                const SpecTables &tm, const SpecTables &tn, hipStream_t s);
+int launch_dct2(const float *x, float *out, float *scratch, int m, int n, SpecView tm,
+                SpecView tn, hipStream_t s);
 // out = inverse along x (kind_x) then along y (kind_y); scale 4
 int launch_inverse2(const float *x, float *out, float *scratch, int m, int n, InvKind kind_x,
-                    InvKind kind_y, const SpecTables &tm, const SpecTables &tn, hipStream_t s);
+                    InvKind kind_y, SpecView tm, SpecView tn, hipStream_t s);
+// The four leaf transforms of the reference (which: 0 dct2_fft2_forward,
+// 1 idct2_fft2_forward, 2 idct_idxst_forward, 3 idxst_idct_forward).
+int launch_spectral_leaf(const float *x, float *out, float *scratch, int m, int n, int which,
+                         SpecView tm, SpecView tn, hipStream_t s);
 
 // Fused potential pipeline (FFT form).
 struct PotentialBufs {

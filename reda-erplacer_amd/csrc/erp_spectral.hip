@@ -605,39 +605,42 @@ int set_lds_limit(const void *fn, size_t bytes) {
 
 }  // namespace
 
+int launch_make_tables(float2 *fft_tw, float2 *dct_tw, int L, hipStream_t s) {
+    hipLaunchKernelGGL(k_make_tables, dim3((L + 255) / 256), dim3(256), 0, s, fft_tw, dct_tw, L);
+    ERP_HIP_TRY(hipGetLastError());
+    return ERP_OK;
+}
+
 int make_spec_tables(SpecTables &t, int L, hipStream_t s) {
     t.L = L;
     int rc = t.fft_tw.alloc(sizeof(float2) * (size_t)(L / 2));
     if (rc) return rc;
     if ((rc = t.dct_tw.alloc(sizeof(float2) * (size_t)L))) return rc;
-    hipLaunchKernelGGL(k_make_tables, dim3((L + 255) / 256), dim3(256), 0, s,
-                       t.fft_tw.as<float2>(), t.dct_tw.as<float2>(), L);
-    ERP_HIP_TRY(hipGetLastError());
-    return ERP_OK;
+    return launch_make_tables(t.fft_tw.as<float2>(), t.dct_tw.as<float2>(), L, s);
 }
 
 int potential_partials(int m) { return m / (2 * kRowPairs); }
 
-int launch_dct2(const float *x, float *out, float *scratch, int m, int n, const SpecTables &tm,
-                const SpecTables &tn, hipStream_t s) {
+int launch_dct2(const float *x, float *out, float *scratch, int m, int n, SpecView tm,
+                SpecView tn, hipStream_t s) {
     const int F = kRowPairs, logN = ilog2(n), logM = ilog2(m), G = col_pairs(m);
     size_t lr = row_lds(n, F);
     int rc = set_lds_limit((const void *)k_row_dct, lr);
     if (rc) return rc;
     hipLaunchKernelGGL(k_row_dct, dim3(m / (2 * F)), dim3(kBlock), lr, s, x, scratch, n, logN, F,
-                       tn.fft_tw.as<float2>(), tn.dct_tw.as<float2>());
+                       tn.fft_tw, tn.dct_tw);
     ERP_HIP_TRY(hipGetLastError());
     size_t lc = row_lds(m, G);
     if ((rc = set_lds_limit((const void *)k_col_dct, lc))) return rc;
     hipLaunchKernelGGL(k_col_dct, dim3(n / (2 * G)), dim3(kBlock), lc, s, scratch, out, m, logM,
-                       n, G, tm.fft_tw.as<float2>(), tm.dct_tw.as<float2>(),
+                       n, G, tm.fft_tw, tm.dct_tw,
                        (float)(4.0 / ((double)m * (double)n)));
     ERP_HIP_TRY(hipGetLastError());
     return ERP_OK;
 }
 
 int launch_inverse2(const float *x, float *out, float *scratch, int m, int n, InvKind kind_x,
-                    InvKind kind_y, const SpecTables &tm, const SpecTables &tn, hipStream_t s) {
+                    InvKind kind_y, SpecView tm, SpecView tn, hipStream_t s) {
     const int F = kRowPairs, logN = ilog2(n), logM = ilog2(m), G = col_pairs(m);
     size_t lc = row_lds(m, G);
     const void *colk = kind_x == INV_IDCT ? (const void *)k_col_inv<INV_IDCT>
@@ -646,10 +649,10 @@ int launch_inverse2(const float *x, float *out, float *scratch, int m, int n, In
     if (rc) return rc;
     if (kind_x == INV_IDCT)
         hipLaunchKernelGGL(k_col_inv<INV_IDCT>, dim3(n / (2 * G)), dim3(kBlock), lc, s, x, scratch,
-                           m, logM, n, G, tm.fft_tw.as<float2>(), tm.dct_tw.as<float2>());
+                           m, logM, n, G, tm.fft_tw, tm.dct_tw);
     else
         hipLaunchKernelGGL(k_col_inv<INV_IDXST>, dim3(n / (2 * G)), dim3(kBlock), lc, s, x,
-                           scratch, m, logM, n, G, tm.fft_tw.as<float2>(), tm.dct_tw.as<float2>());
+                           scratch, m, logM, n, G, tm.fft_tw, tm.dct_tw);
     ERP_HIP_TRY(hipGetLastError());
     size_t lr = row_lds(n, F);
     const void *rowk = kind_y == INV_IDCT ? (const void *)k_row_inv<INV_IDCT>
@@ -657,12 +660,23 @@ int launch_inverse2(const float *x, float *out, float *scratch, int m, int n, In
     if ((rc = set_lds_limit(rowk, lr))) return rc;
     if (kind_y == INV_IDCT)
         hipLaunchKernelGGL(k_row_inv<INV_IDCT>, dim3(m / (2 * F)), dim3(kBlock), lr, s, scratch,
-                           out, n, logN, F, tn.fft_tw.as<float2>(), tn.dct_tw.as<float2>());
+                           out, n, logN, F, tn.fft_tw, tn.dct_tw);
     else
         hipLaunchKernelGGL(k_row_inv<INV_IDXST>, dim3(m / (2 * F)), dim3(kBlock), lr, s, scratch,
-                           out, n, logN, F, tn.fft_tw.as<float2>(), tn.dct_tw.as<float2>());
+                           out, n, logN, F, tn.fft_tw, tn.dct_tw);
     ERP_HIP_TRY(hipGetLastError());
     return ERP_OK;
+}
+
+int launch_spectral_leaf(const float *x, float *out, float *scratch, int m, int n, int which,
+                         SpecView tm, SpecView tn, hipStream_t s) {
+    switch (which) {
+    case 0: return launch_dct2(x, out, scratch, m, n, tm, tn, s);
+    case 1: return launch_inverse2(x, out, scratch, m, n, INV_IDCT, INV_IDCT, tm, tn, s);
+    case 2: return launch_inverse2(x, out, scratch, m, n, INV_IDCT, INV_IDXST, tm, tn, s);
+    case 3: return launch_inverse2(x, out, scratch, m, n, INV_IDXST, INV_IDCT, tm, tn, s);
+    default: return fail(ERP_ERR_INVALID, "unknown transform");
+    }
 }
 
 int launch_potential_fft(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
