@@ -790,10 +790,12 @@ int erp_compute_hpwl_gamma(erp_ctx *ctx, double gamma, double *hpwl) {
     if (!(gamma > 0)) return fail(ERP_ERR_INVALID, "gamma must be positive");
     DeviceGuard g(ctx->device);
     if (g.rc) return g.rc;
+    // the copy reads `gamma` from this frame: it completes before the call returns
     ERP_HIP_TRY(hipMemcpyAsync(ctx->gamma_override.p, &gamma, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     int rc = phase_hpwl(ctx, ctx->djob_gamma.as<JobDesc>());
-    if (rc || !hpwl) return rc;
-    return read_scalar(ctx, offsetof(DevState, hpwl), hpwl);
+    if (!rc && hpwl) return read_scalar(ctx, offsetof(DevState, hpwl), hpwl);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess && !rc) rc = fail(ERP_ERR_HIP, "hipStreamSynchronize failed");
+    return rc;
 }
 
 int erp_compute_potential(erp_ctx *ctx, double *energy, double *overflow) {

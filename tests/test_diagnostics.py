@@ -184,3 +184,61 @@ def test_logged_run_matches_graph_run(gpu_lib, caplog):
         for p in parts[1:]:
             v = p.split(" ")[1]
             assert len(v) >= 8 and np.isfinite(np.float32(float(v)))
+
+
+@pytest.mark.gpu
+def test_logged_lines_and_density_png_match_oracle(gpu_lib, caplog, tmp_path):
+    """The GPU run's [000]/[010]/[020] lines against log_line of the FP64
+    oracle's own scalars at the same iterations (placement.rs:213-223: HPWL and
+    overflow of the step, the weight after the Nesterov step, the gamma the
+    step used), field by field; and the step-0 density PNG of the GPU map
+    against draw_density_map of the oracle's map (epotential.rs:200-223)."""
+    import oracle_lib as O
+    from erplacer import lib as L
+    from erplacer import workloads as W
+    from erplacer.placement import Placement
+    d, g, _, td = W.make("small")
+    ref = O.Placer(d, g, target_density=td)
+    gamma = ref.setup()["gamma"]
+    expect = {}
+    for k in range(21):
+        r = ref.step()
+        if k % 10 == 0:
+            # the weight is logged after the Nesterov step and before the
+            # scheduler update; the first update leaves it unchanged (scheduler.rs:
+            # 102-113), later ones multiply it, so take it from the step before
+            dw = r["density_weight"] if k == 0 else prev_dw
+            expect[k] = D.log_line(k, r["hpwl"], r["overflow"], dw, gamma)
+        gamma = r["gamma"]
+        prev_dw = r["density_weight"]
+    with Placement(d, g, target_density=td) as P:
+        with caplog.at_level(logging.INFO, logger="erplacer"):
+            D.run_logged(P, 21)
+    heads = [r.getMessage() for r in caplog.records
+             if r.name == "erplacer" and r.getMessage().startswith("[")]
+    assert len(heads) == 3
+    for line in heads:
+        k = int(line[1:4])
+        want = expect[k]
+        gv = [p.split(" ")[1] for p in line.split(" - ")[1:]]
+        wv = [p.split(" ")[1] for p in want.split(" - ")[1:]]
+        assert [p.split(" ")[0] for p in line.split(" - ")] == [p.split(" ")[0] for p in want.split(" - ")]
+        for a, b in zip(gv, wv):
+            assert len(a) >= 8 and len(b) >= 8
+            assert abs(float(a) / float(b) - 1) < 5e-3, (line, want)
+        if k == 0:
+            # before the trajectories part, the f32 strings agree to 4 digits
+            for a, b in zip(gv, wv):
+                assert abs(float(a) / float(b) - 1) < 1e-4, (line, want)
+    # step-0 density map PNG
+    ref0 = O.Placer(d, g, target_density=td)
+    ref0.step()
+    with Placement(d, g, target_density=td) as P:
+        P.compute_potential()
+        gm = P.map(L.MAP_DENSITY)
+    pa = D.draw_density_map(gm, 0, out_dir=str(tmp_path / "gpu"))
+    pb = D.draw_density_map(ref0.map(O.MAP_DENSITY).astype(np.float32), 0, out_dir=str(tmp_path / "ref"))
+    a, b = D.read_png(pa), D.read_png(pb)
+    assert a.shape == b.shape == (g, g, 3)
+    diff = np.abs(a.astype(int) - b.astype(int))
+    assert diff.max() <= 1 and np.mean(diff.any(axis=-1)) < 1e-3

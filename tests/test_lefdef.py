@@ -171,3 +171,87 @@ def test_lefdef_design_places(gpu_lib):
         info = P.run(30)
     assert info["iteration"] == 30
     assert np.isfinite(info["hpwl"]) and np.isfinite(info["overflow"])
+
+
+def _generated_lefdef(n_mov=3000, n_fix=4, n_nets=3200, seed=23):
+    """A LEF/DEF pair of standard cells on 0.2 x 2.0 um sites and a few blocks,
+    movable cells at random legal sites, nets of 2-6 pins."""
+    rng = np.random.default_rng(seed)
+    lef = ["UNITS DATABASE MICRONS 2000 ; END UNITS"]
+    for k in range(1, 9):
+        lef.append(f"MACRO C{k} CLASS CORE ; SIZE {0.2 * k:.1f} BY 2.0 ;")
+        for p in range(3):
+            lef.append(f"PIN P{p} PORT LAYER M1 ; RECT {0.02 + 0.05 * p:.2f} 0.4 "
+                       f"{0.06 + 0.05 * p:.2f} 0.8 ; END END P{p}")
+        lef.append(f"END C{k}")
+    lef.append("MACRO B CLASS BLOCK ; SIZE 30 BY 24 ; PIN P0 PORT LAYER M1 ; RECT 1 1 2 2 ; "
+               "END END P0 PIN P1 PORT LAYER M1 ; RECT 10 20 11 21 ; END END P1 "
+               "PIN P2 PORT LAYER M1 ; RECT 25 5 26 6 ; END END P2 END B")
+    die = 110_000                                     # DEF units (1000 per micron)
+    comps = []
+    for i in range(n_mov):
+        k = int(rng.integers(1, 9))
+        comps.append(f"- c{i} C{k} + PLACED ( {int(rng.integers(0, 540)) * 200} "
+                     f"{int(rng.integers(0, 54)) * 2000} ) N ;")
+    for j in range(n_fix):
+        comps.append(f"- b{j} B + FIXED ( {10000 + 50000 * (j % 2)} "
+                     f"{10000 + 50000 * (j // 2)} ) N ;")
+    names = [f"c{i}" for i in range(n_mov)] + [f"b{j}" for j in range(n_fix)]
+    nets = []
+    for e in range(n_nets):
+        k = int(rng.choice([2, 2, 2, 3, 3, 4, 5, 6]))
+        base = int(rng.integers(0, n_mov))
+        members = {min(n_mov - 1, base + int(rng.integers(0, 40))) for _ in range(k)}
+        if rng.random() < 0.02:
+            members.add(n_mov + int(rng.integers(0, n_fix)))
+        if len(members) < 2:
+            members.add((base + 1) % n_mov)
+        nets.append(f"- n{e} " + " ".join(f"( {names[m]} P{int(rng.integers(0, 3))} )"
+                                          for m in sorted(members)) + " ;")
+    deff = "\n".join(["UNITS DISTANCE MICRONS 1000 ;", f"DIEAREA ( 0 0 ) ( {die} {die} ) ;",
+                      f"COMPONENTS {len(comps)} ;", *comps, "END COMPONENTS",
+                      f"NETS {len(nets)} ;", *nets, "END NETS"])
+    return "\n".join(lef), deff
+
+
+@pytest.mark.gpu
+def test_lefdef_design_tracks_oracle(gpu_lib):
+    """A parsed LEF/DEF design (3000 standard cells, 4 blocks) placed on the GPU
+    against the FP64 oracle on the same parsed design (main.rs:68-93): step 0 to
+    rounding, the first ten steps within the tracking bounds of the generator's
+    designs, and the density map bit for bit at step 0."""
+    import math
+
+    import oracle_lib as O
+    from erplacer import lib as L
+    from erplacer.placement import Placement
+    lef, deff = _generated_lefdef()
+    d = LD.read_db(lef, deff)
+    assert d.num_movable == 3000 and d.num_nodes == 3004 and d.n_2nets > 0
+    ref = O.Placer(d, 64)
+    with Placement(d, 64) as P:
+        P.set_keep_density(True)
+        x0, _ = P.positions()
+        rx0, _ = ref.positions()
+        assert np.array_equal(x0.astype(np.float64), rx0)
+        h = P.compute_hpwl()
+        _, ovf = P.compute_potential()
+        r = ref.step()
+        assert math.isclose(h, r["hpwl"], rel_tol=1e-5) and abs(ovf - r["overflow"]) < 2e-5
+        q, _, scale = P.density_fixed()
+        st = P.setup()
+        C = d.num_movable
+        y0 = P.positions()[1]
+        q_ref = O.density_map_fx64(x0[:C], y0[:C], st["cw"][:C], st["ch"][:C], st["ox"][:C],
+                                   st["oy"][:C], st["ratio"][:C], 64, 64, st["bw"], st["bh"],
+                                   scale, sfx="_f32")
+        assert np.array_equal(q, q_ref)
+        P.nesterov_step()
+        P.scheduler_update()
+        P.set_keep_density(False)
+        for k in range(1, 10):
+            a = P.step()
+            b = ref.step()
+            assert math.isclose(a["hpwl"], b["hpwl"], rel_tol=1e-3), k
+            assert abs(a["overflow"] - b["overflow"]) < 1e-3, k
+        assert L.SPECTRAL_FFT == 0
