@@ -259,7 +259,20 @@ def serving_bench(reqs, rates, check_every, max_active, max_iterations):
     }
 
 
-def end_state(workload, info, hpwl_dbu):
+def traced_run(P, iterations, every=50):
+    """The workload from iteration 0 in chunks of `every` (bit-identical to one
+    run): (iteration, hpwl, overflow) after each chunk."""
+    P.reset()
+    out, done = [], 0
+    while done < iterations:
+        k = min(every, iterations - done)
+        r = P.run(k)
+        done += k
+        out.append((done, r["hpwl"], r["overflow"]))
+    return out
+
+
+def end_state(workload, info, hpwl_dbu, trace=None):
     """The end state of the whole workload against the FP64 oracle's runs of the
     same workload (tests/golden/<workload>_band.json, written by the oracle on
     the CPU): the unperturbed FP64 run and the spread of its one-ulp
@@ -290,6 +303,24 @@ def end_state(workload, info, hpwl_dbu):
         "hpwl_z_in_band": round(z, 3), "overflow_z_in_band": round(zo, 3),
         "within_band_3sigma": bool(abs(z) <= 3 and abs(zo) <= 3),
     })
+    if trace:
+        # the GPU's HPWL every 50 iterations against the FP64 runs' envelope
+        f64 = [r for r in band["runs"] if r["sfx"] == "_f64"]
+        env = {}
+        for r in f64:
+            for it, h, _ in r["trace"]:
+                lo, hi = env.get(it, (h, h))
+                env[it] = (min(lo, h), max(hi, h))
+        rows = []
+        for it, h, o in trace:
+            if it in env:
+                lo, hi = env[it]
+                mid = 0.5 * (lo + hi)
+                rows.append({"iteration": it, "hpwl": h, "overflow": round(o, 5),
+                             "fp64_min": lo, "fp64_max": hi,
+                             "rel_to_fp64_mid": round(h / mid - 1, 5),
+                             "inside": bool(lo <= h <= hi)})
+        out["trace_vs_fp64"] = rows
     return out
 
 
@@ -402,6 +433,10 @@ def main():
         full_ips = iters_cfg / (time.perf_counter() - t0)
         full_dbu = P.final_hpwl_dbu()
 
+    trace = traced_run(P, iters_cfg)
+    if trace[-1][1] != full_info["hpwl"]:
+        raise RuntimeError("the chunked run of the workload differs from the single run")
+
     # per-kernel device time with HIP events over a second timed region: the
     # workload's full iteration count, so the averages cover the clustered start
     # and the spread end alike (what a kernel trace of the whole run averages)
@@ -480,7 +515,7 @@ def main():
         "final": {"hpwl": info["hpwl"], "overflow": info["overflow"], "hpwl_dbu": hpwl_dbu,
                   "iteration": info["iteration"]},
         "full_workload": {"iterations": iters_cfg, "it_per_s": round(full_ips, 2),
-                          "end_state": end_state(args.workload, full_info, full_dbu)},
+                          "end_state": end_state(args.workload, full_info, full_dbu, trace)},
         "roofline": roof, "kernels": kernels, "cpu_baseline": cpu, "serving": serving,
     }
     print(json.dumps(out), flush=True)

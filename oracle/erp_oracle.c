@@ -21,16 +21,34 @@
 #define M_SQRT2 1.41421356237309504880
 #endif
 
+/* REAL: the element type of every array; SCAL: the scalars of the loop (step
+ * size, momentum, density weight, gamma, scheduler state) and the sums that
+ * produce them (HPWL, overflow, energy, the step-size and L1 dot products). */
 #define REAL double
+#define SCAL double
 #define SFX _f64
 #include "erp_oracle_impl.inc"
 #undef REAL
+#undef SCAL
 #undef SFX
 
 #define REAL float
+#define SCAL float
 #define SFX _f32
 #include "erp_oracle_impl.inc"
 #undef REAL
+#undef SCAL
+#undef SFX
+
+/* f32 arrays with FP64 scalars and sums: the precision split of the GPU path
+ * (f32 coordinates and per-element arithmetic as the reference binary, FP64
+ * step-size dot products, scheduler state, HPWL, overflow and energy sums). */
+#define REAL float
+#define SCAL double
+#define SFX _f32d
+#include "erp_oracle_impl.inc"
+#undef REAL
+#undef SCAL
 #undef SFX
 
 /* Fixed-point scale for the density accumulators: the largest power of two S

@@ -7,11 +7,14 @@
  *
  * It restates, function by function, the reference's Rust path
  * (giammirove/reda-erplacer, src/ directory); each definition in erp_oracle_impl.inc
- * cites the file:line it follows.  Every function exists twice:
+ * cites the file:line it follows.  Every function exists three times:
  *   *_f64  -- the FP64 parity oracle (run with nthreads=1: deterministic);
  *   *_f32  -- the reference binary's own precision (main.rs:69 reads f32), used
  *             as the timed CPU baseline with nthreads = host cores, parallelised
- *             with OpenMP the way the reference parallelises with Rayon.
+ *             with OpenMP the way the reference parallelises with Rayon;
+ *   *_f32d -- f32 arrays and per-element arithmetic with FP64 scalars and sums
+ *             (step size, momentum, weight, gamma; HPWL, overflow, energy and
+ *             dot-product reductions): the precision split the GPU path uses.
  *
  * The *_fx64 functions are not in the reference: they restate, on the CPU, the
  * deterministic 64-bit fixed-point density accumulation the GPU path uses in
@@ -128,6 +131,7 @@ typedef struct {
 
 ERO_DECL(_f64, double)
 ERO_DECL(_f32, float)
+ERO_DECL(_f32d, float)   /* f32 arrays, FP64 scalars and sums (the GPU's split) */
 
 /* maps for ero_get_map */
 enum { ERO_MAP_DENSITY = 0, ERO_MAP_AUV = 1, ERO_MAP_FIELD_X = 2, ERO_MAP_FIELD_Y = 3,

@@ -6,7 +6,9 @@ As make_small_band.py, with fewer runs (each takes ~10 minutes single-threaded):
 the unperturbed FP64 run with the density map summed in float and in fixed
 point, six runs with one FP64 ulp added to one movable x coordinate before an
 early iteration, and two f32 fixed-point runs (the reference binary's
-precision).  Each run also records HPWL and overflow every 50 iterations.
+precision), and one run with f32 arrays and FP64 scalars and sums (the GPU
+path's precision split).  Each run also records HPWL and overflow every 50
+iterations.
 
     python tests/golden/make_large_band.py        # ~15 minutes on 8 cores, ~16 GB
 """
@@ -56,6 +58,8 @@ def specs():
     for _ in range(2):
         out.append(dict(sfx="_f32", fixed_point=True, cell=int(rng.integers(0, 1_000_000)),
                         iteration=int(rng.integers(1, 50))))
+    # the GPU path's precision split: f32 arrays, FP64 scalars and sums
+    out.append(dict(sfx="_f32d", fixed_point=True, cell=None, iteration=None))
     return out
 
 
@@ -73,9 +77,12 @@ def main():
         runs = list(ex.map(run_one, specs()))
     f64 = [r for r in runs if r["sfx"] == "_f64"]
     f32 = [r for r in runs if r["sfx"] == "_f32"]
+    f32d = [r for r in runs if r["sfx"] == "_f32d"]
     doc = dict(workload="large", iterations=800, perturbation="one FP64 ulp of one movable x "
                "coordinate before the given iteration", seed=SEED,
-               fp64=summary(f64), f32_fixed_point=summary(f32), runs=runs)
+               fp64=summary(f64), f32_fixed_point=summary(f32),
+               f32_fp64_scalars=dict(hpwl=f32d[0]["hpwl"], overflow=f32d[0]["overflow"]),
+               runs=runs)
     with open(OUT, "w") as f:
         json.dump(doc, f, indent=1)
     for k in ("fp64", "f32_fixed_point"):

@@ -64,7 +64,7 @@ def lib():
             build()
         _lib = C.CDLL(LIB_PATH)
         _lib.ero_fx_scale.restype = C.c_double
-        for sfx in ("_f64", "_f32"):
+        for sfx in ("_f64", "_f32", "_f32d"):
             getattr(_lib, "ero_create" + sfx).restype = C.c_void_p
             getattr(_lib, "ero_compute_hpwl" + sfx).restype = C.c_double if sfx == "_f64" else C.c_float
             getattr(_lib, "ero_wa_value" + sfx).restype = C.c_double if sfx == "_f64" else C.c_float

@@ -115,3 +115,23 @@ def test_f32_coordinates_part_at_step_three(small):
     assert np.median(np.abs(xs[1] - xs[0]) / ulp) < 16
     # and step 3 by thousands
     assert np.median(np.abs(xs[3] - xs[2]) / ulp) > 1000
+
+
+def test_large_band_is_wider_than_half_a_percent():
+    """tests/golden/large_band.json (make_large_band.py: eight FP64 runs of the
+    1M-cell, 1024^2, 800-iteration workload, one-ulp perturbations and float /
+    fixed-point maps): the final HPWL spreads by several percent, so the metric's
+    'within 0.5 % of the FP64 run' is inside the oracle's own noise there too."""
+    with open(os.path.join(os.path.dirname(BAND), "large_band.json")) as f:
+        b = json.load(f)
+    s = b["fp64"]
+    assert b["iterations"] == 800 and s["n"] == 8
+    assert s["hpwl_std"] / s["hpwl_mean"] > 0.01
+    assert s["hpwl_max"] / s["hpwl_min"] - 1 > 0.05
+    assert s["overflow_max"] - s["overflow_min"] > 0.01
+    runs = [r for r in b["runs"] if r["sfx"] == "_f64"]
+    # every FP64 run agrees with the others to 1e-3 at iteration 50 (the runs
+    # part later), so the spread is grown, not seeded
+    h50 = np.array([r["trace"][0][1] for r in runs])
+    assert np.all([r["trace"][0][0] == 50 for r in runs])
+    assert h50.max() / h50.min() - 1 < 1e-3
