@@ -303,9 +303,9 @@ def end_state(workload, info, hpwl_dbu, trace=None):
         "hpwl_z_in_band": round(z, 3), "overflow_z_in_band": round(zo, 3),
         "within_band_3sigma": bool(abs(z) <= 3 and abs(zo) <= 3),
     })
-    if trace:
+    f64 = [r for r in band["runs"] if r["sfx"] == "_f64"]
+    if trace and all("trace" in r for r in f64):
         # the GPU's HPWL every 50 iterations against the FP64 runs' envelope
-        f64 = [r for r in band["runs"] if r["sfx"] == "_f64"]
         env = {}
         for r in f64:
             for it, h, _ in r["trace"]:

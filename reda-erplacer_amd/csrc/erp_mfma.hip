@@ -11,17 +11,25 @@
 // folds its input (x_i +- x_{L-1-i}) and multiplies by the even / odd columns of
 // C; an inverse multiplies even / odd frequencies separately and unfolds
 // (E + O at i, E - O at L-1-i).  Every product is then a pair of
-// (rows x L/2) x (L/2 x cols) GEMMs.
+// (rows x L/2) x (L/2 x cols) GEMMs, K = L/2.
 //
 // Precision: operands are split into bf16 pairs (x = hi + lo, |lo| <= 2^-9 |x|)
 // and each product is hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16 with
 // FP32 accumulation (relative error ~2^-17 per term).  The operator matrices
 // are made once in FP64 and stored split.
 //
-// Layout between the passes (all row-major m x n floats): the row-transformed
-// map and the spectra keep the frequency index de-interleaved (even
-// frequencies in the first half, odd in the second) along every transformed
-// axis, so each half-GEMM reads contiguous blocks.
+// Data flow: every pass writes its result already folded, split into bf16
+// hi / lo and laid out the way the next pass reads it -- four "planes" per map
+// ([product half e / o][hi / lo], K contiguous) -- so the GEMM kernel only
+// copies bf16 tiles into LDS:
+//   k_finalize : q -> rho (f32) and the row pass operand, rows folded along y
+//   FWD_ROW    : -> the column pass operand, folded along x (each workgroup owns
+//                a band of rows and its mirror image, so it can fold)
+//   FWD_COL    : -> the three spectra (a.Wx eps, eps a.Wy, eps eps a.inv), split
+//                by the parity of the x frequency
+//   INV_COL    : -> the three column inverses, unfolded along x, split by the
+//                parity of the y frequency
+//   INV_ROW    : -> xi_x, xi_y (the interleaved field map), phi, energy
 #include <cmath>
 
 #include "erp_internal.h"
@@ -33,6 +41,7 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));   // 16 bytes of operand tile
 
 enum { FWD_ROW = 0, FWD_COL = 1, INV_COL = 2, INV_ROW = 3 };
 // operator planes of one length (each (L/2)^2, hi then lo):
@@ -40,28 +49,46 @@ enum { FWD_ROW = 0, FWD_COL = 1, INV_COL = 2, INV_ROW = 3 };
 //   inverse  [i][k'] = C[i][2k' + p], S[...]  (CE, CO, SE, SO)
 enum { T_FE = 0, T_FO = 1, T_CE = 2, T_CO = 3, T_SE = 4, T_SO = 5, T_COUNT = 6 };
 
-constexpr int kBM = 64;              // output rows per workgroup (each half product)
-constexpr int kBN = 32;              // output columns per workgroup (each half product)
-constexpr int kBK = 64;              // K per stage (two MFMA steps of 32)
-constexpr int kLd = kBK + 8;         // LDS row length in bf16 (16-byte pad)
-constexpr int kAPlane = kBM * kLd;
-constexpr int kBPlane = kBN * kLd;
-constexpr int kStage = 4 * kAPlane + 4 * kBPlane;   // planes: [product e/o][hi/lo]
+constexpr int kTile = 64;               // output rows and columns of a workgroup (per product half)
+constexpr int kKb = 32;                 // K per stage: one 16x16x32 MFMA step
+constexpr int kPlane = kTile * kKb;     // bf16 per LDS operand tile (4 KB)
+constexpr int kTp = kTile + 1;          // padded row of the f32 epilogue tiles
 
-struct MfmaArgs {
-    const float *src[3];
-    float *dst[3];
-    int dst_stride[3];        // INV_ROW: element stride of each output (2 for the field pair)
-    const __bf16 *tab;        // operator planes of the transformed length
-    int H;                    // half length (the K of every product)
+// Four planes of one split map: [half][hi/lo], each `rows` x `ld` bf16, K contiguous.
+struct Split {
+    __bf16 *base;
+    int rows, ld;
+};
+__host__ __device__ inline __bf16 *plane(const Split &s, int half, int hl) {
+    return s.base + (size_t)(2 * half + hl) * s.rows * s.ld;
+}
+
+struct GemmArgs {
+    const __bf16 *op;         // operator planes of the transformed length
+    int H;                    // half length: the K of every product
+    Split in[3];              // data operand per transform z
+    Split out[3];             // FWD_ROW / FWD_COL / INV_COL: split result per z
+    float *dst[3];            // INV_ROW result per z; FWD_COL without spectra: natural a
+    int dst_stride[3];        // INV_ROW: element stride (2 for a component of the field map)
     int m, n;
-    int sine_mask;            // bit z: launch z uses the sine operator
-    int spectral;             // FWD_COL: write the three spectra (else plain a)
+    int sine_mask;            // bit z: transform z uses the sine operator
+    int spectral;             // FWD_COL: write the three spectra (else natural a into dst[0])
     float scale, two_pi_over_m, two_pi_over_n, wv_div;
     float *auv;               // FWD_COL: a in natural order (nullable)
-    const float *rho;         // INV_ROW: density for the energy of launch z = 2
+    const float *rho;         // INV_ROW: density for the energy of z = 2
     double *energy_part;
 };
+
+// element z of a kernel-argument array, selected without dynamic indexing (which
+// would copy the arguments to scratch)
+template <class T>
+__device__ __forceinline__ T pick3(const T (&v)[3], int z) {
+    return z == 0 ? v[0] : (z == 1 ? v[1] : v[2]);
+}
+
+__device__ __forceinline__ const __bf16 *op_plane(const GemmArgs &a, int t, int hl) {
+    return a.op + (size_t)(2 * t + hl) * a.H * a.H;
+}
 
 __device__ __forceinline__ void split8(const float *v, bf16x8 &hi, bf16x8 &lo) {
 #pragma unroll
@@ -72,129 +99,11 @@ __device__ __forceinline__ void split8(const float *v, bf16x8 &hi, bf16x8 &lo) {
     }
 }
 
-__device__ __forceinline__ const __bf16 *plane(const MfmaArgs &a, int t, int hl) {
-    return a.tab + (size_t)(2 * t + hl) * a.H * a.H;
-}
-
-// Global -> register staging of one K stage (two stages are in flight).
-struct Stage {
-    float4 d[8];          // data operand
-    bf16x8 o[8];          // operator operand
-};
-
-// Data as the A operand (rows x K, K contiguous): FWD_ROW folds column k with
-// n-1-k, INV_ROW takes the two de-interleaved halves.  Thread: one row, 16 K.
-template <int MODE>
-__device__ __forceinline__ void load_a_data(const MfmaArgs &a, const float *src, int r0, int k0,
-                                            Stage &s) {
-    const int row = threadIdx.x >> 2, kk = (threadIdx.x & 3) * 16;
-    const float *p = src + (size_t)(r0 + row) * a.n;
-    const float4 *x0 = reinterpret_cast<const float4 *>(p + k0 + kk);
-    const float4 *x1 = MODE == FWD_ROW ? reinterpret_cast<const float4 *>(p + a.n - 16 - k0 - kk)
-                                       : reinterpret_cast<const float4 *>(p + a.H + k0 + kk);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        s.d[q] = x0[q];
-        s.d[4 + q] = x1[q];
-    }
-}
-template <int MODE>
-__device__ __forceinline__ void store_a_data(const Stage &s, __bf16 *lds) {
-    const int row = threadIdx.x >> 2, kk = (threadIdx.x & 3) * 16;
-    const float *lo = reinterpret_cast<const float *>(&s.d[0]);
-    const float *hi = reinterpret_cast<const float *>(&s.d[4]);
-#pragma unroll
-    for (int h8 = 0; h8 < 2; ++h8) {
-        float e[8], o[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int jj = 8 * h8 + j;
-            if (MODE == FWD_ROW) { e[j] = lo[jj] + hi[15 - jj]; o[j] = lo[jj] - hi[15 - jj]; }
-            else { e[j] = lo[jj]; o[j] = hi[jj]; }
-        }
-        const int at = row * kLd + kk + 8 * h8;
-        bf16x8 vh, vl;
-        split8(e, vh, vl);
-        *reinterpret_cast<bf16x8 *>(lds + 0 * kAPlane + at) = vh;
-        *reinterpret_cast<bf16x8 *>(lds + 1 * kAPlane + at) = vl;
-        split8(o, vh, vl);
-        *reinterpret_cast<bf16x8 *>(lds + 2 * kAPlane + at) = vh;
-        *reinterpret_cast<bf16x8 *>(lds + 3 * kAPlane + at) = vl;
-    }
-}
-
-// Data as the B operand (K x cols, K along rows of the map): FWD_COL folds row
-// k with 2H-1-k, INV_COL takes the two de-interleaved halves.  Stored
-// transposed (K contiguous per column).  Thread: two adjacent K rows, 4 columns.
-template <int MODE>
-__device__ __forceinline__ void load_b_data(const MfmaArgs &a, const float *src, int c0, int k0,
-                                            Stage &s) {
-    const int k2 = (threadIdx.x >> 3) * 2, cl = (threadIdx.x & 7) * 4;
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int r = k0 + k2 + q;
-        const int r1 = MODE == FWD_COL ? 2 * a.H - 1 - r : a.H + r;
-        s.d[q] = *reinterpret_cast<const float4 *>(src + (size_t)r * a.n + c0 + cl);
-        s.d[2 + q] = *reinterpret_cast<const float4 *>(src + (size_t)r1 * a.n + c0 + cl);
-    }
-}
-template <int MODE>
-__device__ __forceinline__ void store_b_data(const Stage &s, __bf16 *lds) {
-    const int k2 = (threadIdx.x >> 3) * 2, cl = (threadIdx.x & 7) * 4;
-    const float *x0 = reinterpret_cast<const float *>(&s.d[0]);
-    const float *x1 = reinterpret_cast<const float *>(&s.d[1]);
-    const float *y0 = reinterpret_cast<const float *>(&s.d[2]);
-    const float *y1 = reinterpret_cast<const float *>(&s.d[3]);
-    __bf16 *B = lds + 4 * kAPlane;
-    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const float e0 = MODE == FWD_COL ? x0[q] + y0[q] : x0[q];
-        const float e1 = MODE == FWD_COL ? x1[q] + y1[q] : x1[q];
-        const float o0 = MODE == FWD_COL ? x0[q] - y0[q] : y0[q];
-        const float o1 = MODE == FWD_COL ? x1[q] - y1[q] : y1[q];
-        bf16x2 eh, el, oh, ol;
-        eh[0] = (__bf16)e0; eh[1] = (__bf16)e1;
-        el[0] = (__bf16)(e0 - (float)eh[0]); el[1] = (__bf16)(e1 - (float)eh[1]);
-        oh[0] = (__bf16)o0; oh[1] = (__bf16)o1;
-        ol[0] = (__bf16)(o0 - (float)oh[0]); ol[1] = (__bf16)(o1 - (float)oh[1]);
-        const int at = (cl + q) * kLd + k2;
-        *reinterpret_cast<bf16x2 *>(B + 0 * kBPlane + at) = eh;
-        *reinterpret_cast<bf16x2 *>(B + 1 * kBPlane + at) = el;
-        *reinterpret_cast<bf16x2 *>(B + 2 * kBPlane + at) = oh;
-        *reinterpret_cast<bf16x2 *>(B + 3 * kBPlane + at) = ol;
-    }
-}
-
-// Operator as the A operand: 64 rows x 64 K of four planes (thread: one row of
-// one plane).
-__device__ __forceinline__ void load_a_op(const MfmaArgs &a, int te, int r0, int k0, Stage &s) {
-    const int pl = threadIdx.x >> 6, row = threadIdx.x & 63;
-    const int t = te + (pl >> 1);   // even / odd product
-    const bf16x8 *p = reinterpret_cast<const bf16x8 *>(plane(a, t, pl & 1) + (size_t)(r0 + row) * a.H + k0);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) s.o[q] = p[q];
-}
-__device__ __forceinline__ void store_a_op(const Stage &s, __bf16 *lds) {
-    const int pl = threadIdx.x >> 6, row = threadIdx.x & 63;
-    bf16x8 *d = reinterpret_cast<bf16x8 *>(lds + pl * kAPlane + row * kLd);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) d[q] = s.o[q];
-}
-// Operator as the B operand: 32 columns x 64 K of four planes (thread: half a
-// column row of one plane).
-__device__ __forceinline__ void load_b_op(const MfmaArgs &a, int te, int c0, int k0, Stage &s) {
-    const int pl = threadIdx.x >> 6, col = (threadIdx.x >> 1) & 31, half = threadIdx.x & 1;
-    const int t = te + (pl >> 1);
-    const bf16x8 *p = reinterpret_cast<const bf16x8 *>(plane(a, t, pl & 1) + (size_t)(c0 + col) * a.H + k0 + 32 * half);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) s.o[q] = p[q];
-}
-__device__ __forceinline__ void store_b_op(const Stage &s, __bf16 *lds) {
-    const int pl = threadIdx.x >> 6, col = (threadIdx.x >> 1) & 31, half = threadIdx.x & 1;
-    bf16x8 *d = reinterpret_cast<bf16x8 *>(lds + 4 * kAPlane + pl * kBPlane + col * kLd + 32 * half);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) d[q] = s.o[q];
+// LDS tile of 64 rows x 32 K bf16: 16-byte chunk c of row r at chunk
+// c ^ ((r >> 1) & 3), so the eight lanes of a ds_read_b128 phase (eight rows,
+// one chunk) hit distinct banks.
+__device__ __forceinline__ int swz(int row, int chunk) {
+    return row * kKb + 8 * (chunk ^ ((row >> 1) & 3));
 }
 
 // XCD-aware tile order: consecutive workgroups go round-robin over the 8 XCDs;
@@ -204,167 +113,324 @@ __device__ __forceinline__ int tile_of_block(int b, int nb) {
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(256) k_mfma(MfmaArgs a) {
-    __shared__ __attribute__((aligned(16))) __bf16 lds[kStage];
-    const int z = blockIdx.y;
+struct GemmShape {
+    static constexpr bool BOTH = MODE == INV_COL || MODE == INV_ROW;   // both product halves per workgroup
+    static constexpr int NH = BOTH ? 2 : 1;
+    static constexpr int NPL = 4 * NH;      // tiles per stage: (A, B) x half x (hi, lo)
+    static constexpr int DEPTH = BOTH ? 3 : 4;   // stages of global loads in flight
+};
+
+// One stage's tiles: global -> registers, registers -> LDS (16 bytes per tile per thread).
+template <int NPL>
+__device__ __forceinline__ void load_set(u32x4 (&st)[NPL], const __bf16 *const (&src)[NPL], int koff) {
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) st[q] = *reinterpret_cast<const u32x4 *>(src[q] + koff);
+}
+template <int NPL>
+__device__ __forceinline__ void store_set(const u32x4 (&st)[NPL], __bf16 *buf, int soff) {
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) *reinterpret_cast<u32x4 *>(buf + q * kPlane + soff) = st[q];
+}
+
+// The products of one LDS stage: per half, 2 x 2 MFMA tiles of 16 x 16 per wave,
+// each hi*hi + hi*lo + lo*hi.
+template <int MODE>
+__device__ __forceinline__ void compute_stage(const __bf16 *buf, f32x4 (&acc)[GemmShape<MODE>::NH][2][2],
+                                              int wr, int wc, int lane) {
+    constexpr int NH = GemmShape<MODE>::NH;
+    const int fr = lane & 15, fc = lane >> 4;
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+        const __bf16 *A = buf + (2 * h) * kPlane;
+        const __bf16 *B = buf + (2 * NH + 2 * h) * kPlane;
+        bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int o = swz(wr * 32 + i * 16 + fr, fc);
+            ah[i] = *reinterpret_cast<const bf16x8 *>(A + o);
+            al[i] = *reinterpret_cast<const bf16x8 *>(A + kPlane + o);
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int o = swz(wc * 32 + j * 16 + fr, fc);
+            bh[j] = *reinterpret_cast<const bf16x8 *>(B + o);
+            bl[j] = *reinterpret_cast<const bf16x8 *>(B + kPlane + o);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                acc[h][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[h][i][j], 0, 0, 0);
+                acc[h][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[h][i][j], 0, 0, 0);
+                acc[h][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[h][i][j], 0, 0, 0);
+            }
+    }
+}
+
+// The register sets of the K loop, each a separately named array (a set index
+// is resolved at compile time, so no set needs addressable storage).
+template <int NPL>
+struct Sets {
+    u32x4 s0[NPL], s1[NPL], s2[NPL], s3[NPL];
+    template <int I>
+    __device__ __forceinline__ u32x4 (&get())[NPL] {
+        if constexpr (I == 0) return s0;
+        else if constexpr (I == 1) return s1;
+        else if constexpr (I == 2) return s2;
+        else return s3;
+    }
+};
+
+// Stage T of the K loop (a compile-time recursion): multiply the LDS stage T,
+// store stage T+1 from its register set into the other LDS stage, refill that
+// set with stage T+1+DEPTH.
+template <int MODE, int NK, int T>
+__device__ __forceinline__ void k_stages(Sets<GemmShape<MODE>::NPL> &st,
+                                         const __bf16 *const (&src)[GemmShape<MODE>::NPL], __bf16 *lds,
+                                         int soff, f32x4 (&acc)[GemmShape<MODE>::NH][2][2], int wr,
+                                         int wc, int lane) {
+    if constexpr (T < NK) {
+        constexpr int NPL = GemmShape<MODE>::NPL, DEPTH = GemmShape<MODE>::DEPTH;
+        compute_stage<MODE>(lds + (T & 1) * NPL * kPlane, acc, wr, wc, lane);
+        if constexpr (T + 1 < NK) {
+            store_set<NPL>(st.template get<(T + 1) % DEPTH>(), lds + ((T + 1) & 1) * NPL * kPlane, soff);
+            if constexpr (T + 1 + DEPTH < NK)
+                load_set<NPL>(st.template get<(T + 1) % DEPTH>(), src, (T + 1 + DEPTH) * kKb);
+        }
+        __syncthreads();
+        k_stages<MODE, NK, T + 1>(st, src, lds, soff, acc, wr, wc, lane);
+    }
+}
+
+template <int MODE, int NK, int D>
+__device__ __forceinline__ void k_prologue(Sets<GemmShape<MODE>::NPL> &st,
+                                           const __bf16 *const (&src)[GemmShape<MODE>::NPL]) {
+    if constexpr (D < GemmShape<MODE>::DEPTH && D < NK) {
+        load_set<GemmShape<MODE>::NPL>(st.template get<D>(), src, D * kKb);
+        k_prologue<MODE, NK, D + 1>(st, src);
+    }
+}
+
+// NK: K stages (H / 32).
+template <int MODE, int NK>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_gemm(GemmArgs a) {
+    constexpr bool BOTH = GemmShape<MODE>::BOTH;
+    constexpr int NH = GemmShape<MODE>::NH, NPL = GemmShape<MODE>::NPL, DEPTH = GemmShape<MODE>::DEPTH;
+    __shared__ __attribute__((aligned(16))) __bf16 lds[2 * NPL * kPlane];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;                     // 32 x 32 outputs per wave
+    const int z = BOTH ? (int)blockIdx.y : 0;
+    const int hsel = BOTH ? 0 : (int)blockIdx.y;                 // single-half passes: which half
+    const int m = a.m, n = a.n;
     const bool sine = (a.sine_mask >> z) & 1;
     const int te = (MODE == FWD_ROW || MODE == FWD_COL) ? T_FE : (sine ? T_SE : T_CE);
-    // output grid of the half products: rows x cols
-    const int cols = (MODE == FWD_ROW || MODE == INV_ROW) ? a.n / 2 : a.n;
-    const int ncolt = cols / kBN;
+
+    const int out_cols = (MODE == FWD_ROW || MODE == INV_ROW) ? n / 2 : n;
+    const int nct = out_cols / kTile;
     const int tile = tile_of_block(blockIdx.x, gridDim.x);
-    const int r0 = (tile / ncolt) * kBM, c0 = (tile % ncolt) * kBN;
-    const float *src = a.src[z];
-    const int nk = a.H / kBK;
+    const int rt = tile / nct, ct = tile % nct;
+    const int r0 = rt * kTile, c0 = ct * kTile;
 
-    auto load = [&](int ks, Stage &st) {
-        const int k0 = ks * kBK;
-        if (MODE == FWD_ROW || MODE == INV_ROW) {
-            load_a_data<MODE>(a, src, r0, k0, st);
-            load_b_op(a, te, c0, k0, st);
-        } else {
-            load_a_op(a, te, r0, k0, st);
-            load_b_data<MODE>(a, src, c0, k0, st);
-        }
-    };
-    auto store = [&](const Stage &st) {
-        if (MODE == FWD_ROW || MODE == INV_ROW) {
-            store_a_data<MODE>(st, lds);
-            store_b_op(st, lds);
-        } else {
-            store_a_op(st, lds);
-            store_b_data<MODE>(st, lds);
-        }
-    };
-
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int lr = lane & 15, lk = 8 * (lane >> 4);
-    f32x4 acc[2][2];
+    // ---- operand rows this thread stages: row `lr` of each 64-row tile, chunk `lc`
+    const int lr = tid >> 2, lc = tid & 3;
+    // FWD_ROW: the workgroup's rows are a band and its mirror image
+    const int arow = MODE == FWD_ROW ? (lr < 32 ? rt * 32 + lr : m - rt * 32 - 64 + lr) : r0 + lr;
+    const int brow = c0 + lr;
+    const __bf16 *src[NPL];
+    const Split din = pick3(a.in, z);
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
+    for (int h = 0; h < NH; ++h) {
+        const int half = BOTH ? h : hsel;
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt) acc[p][rt] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-    auto compute = [&]() {
-#pragma unroll
-        for (int kq = 0; kq < kBK; kq += 32) {
-#pragma unroll
-            for (int p = 0; p < 2; ++p) {
-                const __bf16 *B = lds + 4 * kAPlane + (2 * p) * kBPlane + (wc * 16 + lr) * kLd + lk + kq;
-                const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(B);
-                const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(B + kBPlane);
-#pragma unroll
-                for (int rt = 0; rt < 2; ++rt) {
-                    const __bf16 *A = lds + (2 * p) * kAPlane + (wr * 32 + rt * 16 + lr) * kLd + lk + kq;
-                    const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(A);
-                    const bf16x8 al = *reinterpret_cast<const bf16x8 *>(A + kAPlane);
-                    acc[p][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[p][rt], 0, 0, 0);
-                    acc[p][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[p][rt], 0, 0, 0);
-                    acc[p][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[p][rt], 0, 0, 0);
-                }
+        for (int hl = 0; hl < 2; ++hl) {
+            const Split &d = din;
+            const __bf16 *pa, *pb;
+            if (MODE == FWD_ROW || MODE == INV_ROW) {            // A = data rows, B = operator
+                pa = plane(d, half, hl) + (size_t)arow * d.ld;
+                pb = op_plane(a, te + half, hl) + (size_t)brow * a.H;
+            } else {                                             // A = operator, B = data columns
+                pa = op_plane(a, te + half, hl) + (size_t)arow * a.H;
+                pb = plane(d, half, hl) + (size_t)brow * d.ld;
             }
+            src[2 * h + hl] = pa + 8 * lc;
+            src[2 * NH + 2 * h + hl] = pb + 8 * lc;
         }
+    }
+    const int soff = swz(lr, lc);
+
+    f32x4 acc[NH][2][2];
+#pragma unroll
+    for (int h = 0; h < NH; ++h)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[h][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // K loop over a double-buffered LDS stage, one barrier per stage; the global
+    // loads of stage t go to register set t % DEPTH and stay in flight for DEPTH
+    // stages before they are stored to LDS
+    Sets<NPL> st;
+    k_prologue<MODE, NK, 0>(st, src);
+    store_set<NPL>(st.template get<0>(), lds, soff);
+    if constexpr (DEPTH < NK) load_set<NPL>(st.template get<0>(), src, DEPTH * kKb);
+    __syncthreads();
+    k_stages<MODE, NK, 0>(st, src, lds, soff, acc, wr, wc, lane);
+
+    // ------------------------------------------------------------- epilogues
+    // lane holds C[row = wr*32 + i*16 + 4*(lane/16) + j][col = wc*32 + jj*16 + lane%16]
+    float *T = reinterpret_cast<float *>(lds);
+    auto to_lds = [&](float *dstT, int h) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    dstT[(wr * 32 + i * 16 + 4 * (lane >> 4) + j) * kTp + wc * 32 + jj * 16 + (lane & 15)] =
+                        acc[h][i][jj][j];
     };
 
-    // two register stages in flight; one LDS stage
-    Stage s0, s1;
-    load(0, s0);
-    if (nk > 1) load(1, s1);
-    for (int ks = 0; ks < nk; ks += 2) {
-        __syncthreads();                 // the previous stage has been consumed
-        store(s0);
+    if (MODE == FWD_ROW) {
+        // fold the band with its mirror (rows l and 63 - l) for the column pass:
+        // its operand row = the row-transform column hsel * n/2 + c, K = the row
+        to_lds(T, 0);
         __syncthreads();
-        if (ks + 2 < nk) load(ks + 2, s0);
-        compute();
-        if (ks + 1 < nk) {
-            __syncthreads();
-            store(s1);
-            __syncthreads();
-            if (ks + 3 < nk) load(ks + 3, s1);
-            compute();
+        const int c = tid & 63, lb = 8 * (tid >> 6);
+        float e[8], o[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float v1 = T[(lb + q) * kTp + c], v2 = T[(63 - lb - q) * kTp + c];
+            e[q] = v1 + v2;
+            o[q] = v1 - v2;
         }
-    }
-
-    // ------------------------------------------------------------- epilogue
-    // lane holds P[row = wr*32 + rt*16 + 4*(lane/16) + j][col = wc*16 + lane%16]
-    const int n = a.n, m = a.m;
-    double energy = 0.0;
-    const int col = c0 + wc * 16 + lr;
+        const Split &O = a.out[0];
+        const size_t at = (size_t)(hsel * (n / 2) + c0 + c) * O.ld + rt * 32 + lb;
+        bf16x8 hi, lo;
+        split8(e, hi, lo);
+        *reinterpret_cast<bf16x8 *>(plane(O, 0, 0) + at) = hi;
+        *reinterpret_cast<bf16x8 *>(plane(O, 0, 1) + at) = lo;
+        split8(o, hi, lo);
+        *reinterpret_cast<bf16x8 *>(plane(O, 1, 0) + at) = hi;
+        *reinterpret_cast<bf16x8 *>(plane(O, 1, 1) + at) = lo;
+    } else if (MODE == FWD_COL) {
+        // rows k' (x frequency u = 2k' + half), columns c (de-interleaved y
+        // frequency): the three spectra, split by the parity of u, K = k'
+        to_lds(T, 0);
+        __syncthreads();
+        const int c = tid & 63, kb = 16 * (tid >> 6);
+        const int cg = c0 + c;
+        const int v = cg < n / 2 ? 2 * cg : 2 * (cg - n / 2) + 1;
+        float sp[3][16];
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt) {
+        for (int q = 0; q < 16; ++q) {
+            const int u = 2 * (r0 + kb + q) + hsel;
+            const float av = a.scale * T[(kb + q) * kTp + c];
+            if (a.auv) a.auv[(size_t)u * n + v] = av;
+            if (!a.spectral) {
+                a.dst[0][(size_t)u * n + v] = av;
+                continue;
+            }
+            // frequencies.rs:22-57 in f32
+            const float wu = (float)u * a.two_pi_over_m;
+            const float wv = (float)v * a.two_pi_over_n * a.wv_div;
+            const float den = wu * wu + wv * wv;
+            const float inv = (u == 0 && v == 0) ? 0.f : 1.f / den;
+            const float wx = wu * inv * 0.5f, wy = wv * inv * 0.5f;
+            const float eu = u ? 1.f : 0.5f, ev = v ? 1.f : 0.5f;
+            sp[0][q] = av * wx * ev;
+            sp[1][q] = eu * (av * wy);
+            sp[2][q] = eu * ev * (av * inv);
+        }
+        if (a.spectral) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int row = r0 + wr * 32 + rt * 16 + 4 * (lane >> 4) + j;
-            const float pe = acc[0][rt][j], po = acc[1][rt][j];
-            if (MODE == FWD_ROW) {
-                float *d = a.dst[z] + (size_t)row * n;
-                d[col] = pe;
-                d[n / 2 + col] = po;
-            } else if (MODE == FWD_COL) {
-                // row = k' (frequency pair), col = de-interleaved column position
-                const int v = col < n / 2 ? 2 * col : 2 * (col - n / 2) + 1;
+            for (int zz = 0; zz < 3; ++zz) {
+                const Split &O = a.out[zz];
+                const size_t at = (size_t)cg * O.ld + r0 + kb;
 #pragma unroll
-                for (int p = 0; p < 2; ++p) {
-                    const int u = 2 * row + p;
-                    const float av = a.scale * (p ? po : pe);
-                    const size_t at = (size_t)(p * (m / 2) + row) * n + col;
-                    if (a.auv) a.auv[(size_t)u * n + v] = av;
-                    if (!a.spectral) {
-                        a.dst[0][(size_t)u * n + v] = av;
-                        continue;
-                    }
-                    // frequencies.rs:22-57 in f32
-                    const float wu = (float)u * a.two_pi_over_m;
-                    const float wv = (float)v * a.two_pi_over_n * a.wv_div;
-                    const float den = wu * wu + wv * wv;
-                    const float inv = (u == 0 && v == 0) ? 0.f : 1.f / den;
-                    const float wx = wu * inv * 0.5f, wy = wv * inv * 0.5f;
-                    const float eu = u ? 1.f : 0.5f, ev = v ? 1.f : 0.5f;
-                    a.dst[0][at] = av * wx * ev;
-                    a.dst[1][at] = eu * (av * wy);
-                    a.dst[2][at] = eu * ev * (av * inv);
-                }
-            } else if (MODE == INV_COL) {
-                float *d = a.dst[z];
-                d[(size_t)row * n + col] = pe + po;
-                d[(size_t)(m - 1 - row) * n + col] = sine ? po - pe : pe - po;
-            } else {
-                const int st = a.dst_stride[z];   // 2: a component of the field map (fxy_index)
-                const float x0 = 4.f * (pe + po);
-                const float x1 = 4.f * (sine ? po - pe : pe - po);
-                if (st == 2) {
-                    a.dst[z][2 * fxy_index(row, col, n)] = x0;
-                    a.dst[z][2 * fxy_index(row, n - 1 - col, n)] = x1;
-                } else {
-                    float *d = a.dst[z] + (size_t)row * n;
-                    d[col] = x0;
-                    d[n - 1 - col] = x1;
-                }
-                if (a.energy_part && z == 2) {
-                    const float *rr = a.rho + (size_t)row * n;
-                    energy += (double)(x0 * rr[col]) + (double)(x1 * rr[n - 1 - col]);
+                for (int g = 0; g < 2; ++g) {
+                    bf16x8 hi, lo;
+                    split8(&sp[zz][8 * g], hi, lo);
+                    *reinterpret_cast<bf16x8 *>(plane(O, hsel, 0) + at + 8 * g) = hi;
+                    *reinterpret_cast<bf16x8 *>(plane(O, hsel, 1) + at + 8 * g) = lo;
                 }
             }
         }
-    }
-    if (MODE == INV_ROW) {
+    } else if (MODE == INV_COL) {
+        // unfold along x (rows i and m-1-i); the row pass reads rows, split by
+        // the parity of the y frequency (column half), K = column within it
+        float *TE = T, *TO = T + kTile * kTp;
+        to_lds(TE, 0);
+        to_lds(TO, NH - 1);
+        __syncthreads();
+        const int l = tid >> 2, cb = 16 * (tid & 3);
+        const int i = r0 + l, mi = m - 1 - i;
+        const int cg = c0 + cb;
+        const int q = cg < n / 2 ? 0 : 1, k = cg - q * (n / 2);
+        float y0[16], y1[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const float e = TE[l * kTp + cb + t], o = TO[l * kTp + cb + t];
+            y0[t] = e + o;
+            y1[t] = sine ? o - e : e - o;
+        }
+        const Split O = pick3(a.out, z);
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            bf16x8 hi, lo;
+            split8(&y0[8 * g], hi, lo);
+            const size_t a0 = (size_t)i * O.ld + k + 8 * g, a1 = (size_t)mi * O.ld + k + 8 * g;
+            *reinterpret_cast<bf16x8 *>(plane(O, q, 0) + a0) = hi;
+            *reinterpret_cast<bf16x8 *>(plane(O, q, 1) + a0) = lo;
+            split8(&y1[8 * g], hi, lo);
+            *reinterpret_cast<bf16x8 *>(plane(O, q, 0) + a1) = hi;
+            *reinterpret_cast<bf16x8 *>(plane(O, q, 1) + a1) = lo;
+        }
+    } else {
+        // INV_ROW: unfold along y (columns i and n-1-i), x4; energy of phi
+        double energy = 0.0;
+        const int st = pick3(a.dst_stride, z);
+        float *d = pick3(a.dst, z);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int row = r0 + wr * 32 + i * 16 + 4 * (lane >> 4) + j;
+                    const int col = c0 + wc * 32 + jj * 16 + (lane & 15);
+                    const float pe = acc[0][i][jj][j], po = acc[NH - 1][i][jj][j];
+                    const float x0 = 4.f * (pe + po);
+                    const float x1 = 4.f * (sine ? po - pe : pe - po);
+                    if (st == 2) {
+                        d[2 * fxy_index(row, col, n)] = x0;
+                        d[2 * fxy_index(row, n - 1 - col, n)] = x1;
+                    } else {
+                        d[(size_t)row * n + col] = x0;
+                        d[(size_t)row * n + n - 1 - col] = x1;
+                    }
+                    if (a.energy_part && z == 2) {
+                        const float *rr = a.rho + (size_t)row * n;
+                        energy += (double)(x0 * rr[col]) + (double)(x1 * rr[n - 1 - col]);
+                    }
+                }
         if (a.energy_part && z == 2) {
-            __shared__ double red[256];
-            red[threadIdx.x] = energy;
+            double *red = reinterpret_cast<double *>(lds);
+            red[tid] = energy;
             __syncthreads();
             for (int s = 128; s > 0; s >>= 1) {
-                if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+                if (tid < s) red[tid] += red[tid + s];
                 __syncthreads();
             }
-            if (threadIdx.x == 0) a.energy_part[blockIdx.x] = red[0];
+            if (tid == 0) a.energy_part[blockIdx.x] = red[0];
         }
     }
 }
 
 // Density from the fixed-point accumulators (epotential.rs:271-285): rho,
-// overflow partials per 4 rows, movable accumulators cleared.
-__global__ void __launch_bounds__(1024) k_finalize(PotentialBufs b) {
+// overflow partials per 4 rows, movable accumulators cleared; then the row
+// pass operand: rho folded along y (k with n-1-k), split.
+__global__ void __launch_bounds__(1024) k_finalize(PotentialBufs b, Split rows) {
+    extern __shared__ __attribute__((aligned(16))) float R[];   // 4 rows of rho
     const int n = b.n;
     const size_t base = (size_t)blockIdx.x * 4 * n;
     const longlong2 *__restrict__ qf = reinterpret_cast<const longlong2 *>(b.q_fix + base);
@@ -378,8 +444,11 @@ __global__ void __launch_bounds__(1024) k_finalize(PotentialBufs b) {
         if (!b.keep_q) qm[k] = make_longlong2(0, 0);
         const int64_t t0 = a.x + f.x, t1 = a.y + f.y;
         ovf += (t0 > b.coef_q ? t0 - b.coef_q : 0) + (t1 > b.coef_q ? t1 - b.coef_q : 0);
-        rho[k] = make_float2((float)((double)t0 * b.inv_scale) * b.inv_bin_area,
-                             (float)((double)t1 * b.inv_scale) * b.inv_bin_area);
+        const float2 r = make_float2((float)((double)t0 * b.inv_scale) * b.inv_bin_area,
+                                     (float)((double)t1 * b.inv_scale) * b.inv_bin_area);
+        rho[k] = r;
+        R[2 * k] = r.x;
+        R[2 * k + 1] = r.y;
     }
     __shared__ int64_t red[16];
 #pragma unroll
@@ -391,21 +460,78 @@ __global__ void __launch_bounds__(1024) k_finalize(PotentialBufs b) {
         for (int w = 0; w < 16; ++w) v += red[w];
         b.ovf_part[blockIdx.x] = v;
     }
+    // fold: 4 rows x n/2 columns in runs of 8
+    const int runs = n / 16;                  // per row
+    for (int t = threadIdx.x; t < 4 * runs; t += 1024) {
+        const int rr = t / runs, k = 8 * (t % runs);
+        const float *Rr = R + (size_t)rr * n;
+        float e[8], o[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float x0 = Rr[k + q], x1 = Rr[n - 1 - k - q];
+            e[q] = x0 + x1;
+            o[q] = x0 - x1;
+        }
+        const size_t at = (size_t)(blockIdx.x * 4 + rr) * rows.ld + k;
+        bf16x8 hi, lo;
+        split8(e, hi, lo);
+        *reinterpret_cast<bf16x8 *>(plane(rows, 0, 0) + at) = hi;
+        *reinterpret_cast<bf16x8 *>(plane(rows, 0, 1) + at) = lo;
+        split8(o, hi, lo);
+        *reinterpret_cast<bf16x8 *>(plane(rows, 1, 0) + at) = hi;
+        *reinterpret_cast<bf16x8 *>(plane(rows, 1, 1) + at) = lo;
+    }
 }
 
-// Natural-order spectrum -> de-interleaved layout with the eps factors of the
-// inverse (leaf entry points): out[p(u)][p(v)] = eps_u^eu eps_v^ev X[u][v].
-__global__ void k_deinterleave(const float *__restrict__ x, float *__restrict__ out, int m, int n,
-                               int eps_u, int eps_v) {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)m * n) return;
-    const int u = (int)(idx / n), v = (int)(idx % n);
-    float f = x[idx];
-    if (eps_u && u == 0) f *= 0.5f;
-    if (eps_v && v == 0) f *= 0.5f;
-    const int pu = (u & 1) ? m / 2 + u / 2 : u / 2;
-    const int pv = (v & 1) ? n / 2 + v / 2 : v / 2;
-    out[(size_t)pu * n + pv] = f;
+// Leaf forward transform: a natural m x n map -> the row pass operand.
+__global__ void k_prep_rows(const float *__restrict__ x, Split rows, int m, int n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int runs = n / 16;
+    if (t >= (int64_t)m * runs) return;
+    const int r = (int)(t / runs), k = 8 * (int)(t % runs);
+    const float *xr = x + (size_t)r * n;
+    float e[8], o[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const float x0 = xr[k + q], x1 = xr[n - 1 - k - q];
+        e[q] = x0 + x1;
+        o[q] = x0 - x1;
+    }
+    const size_t at = (size_t)r * rows.ld + k;
+    bf16x8 hi, lo;
+    split8(e, hi, lo);
+    *reinterpret_cast<bf16x8 *>(plane(rows, 0, 0) + at) = hi;
+    *reinterpret_cast<bf16x8 *>(plane(rows, 0, 1) + at) = lo;
+    split8(o, hi, lo);
+    *reinterpret_cast<bf16x8 *>(plane(rows, 1, 0) + at) = hi;
+    *reinterpret_cast<bf16x8 *>(plane(rows, 1, 1) + at) = lo;
+}
+
+// Leaf inverse transforms: a natural-order spectrum X (with the eps factors of
+// the inverse) -> the column pass operand: column c' = de-interleaved y
+// frequency, split by the parity of the x frequency u, K = u / 2.
+__global__ void k_prep_spectrum(const float *__restrict__ x, Split cols, int m, int n, int eps_u,
+                                int eps_v) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int runs = m / 16;                   // runs of 8 k per (column, half)
+    if (t >= (int64_t)n * 2 * runs) return;
+    const int c = (int)(t / (2 * runs)), rem = (int)(t % (2 * runs));
+    const int q = rem / runs, k0 = 8 * (rem % runs);
+    const int v = c < n / 2 ? 2 * c : 2 * (c - n / 2) + 1;
+    float f[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int u = 2 * (k0 + j) + q;
+        float val = x[(size_t)u * n + v];
+        if (eps_u && u == 0) val *= 0.5f;
+        if (eps_v && v == 0) val *= 0.5f;
+        f[j] = val;
+    }
+    const size_t at = (size_t)c * cols.ld + k0;
+    bf16x8 hi, lo;
+    split8(f, hi, lo);
+    *reinterpret_cast<bf16x8 *>(plane(cols, q, 0) + at) = hi;
+    *reinterpret_cast<bf16x8 *>(plane(cols, q, 1) + at) = lo;
 }
 
 // Operator planes of length L in FP64, split to bf16 pairs.
@@ -438,16 +564,17 @@ inline unsigned cdiv(int64_t a, int b) { return (unsigned)((a + b - 1) / b); }
 struct MfmaPlan {
     int m = 0, n = 0;
     DevBuf tab_m, tab_n;          // operator planes (tab_n unused when m == n)
-    DevBuf t0, y0, y1, y2;        // scratch maps for the leaf entry points / inverses
+    DevBuf t0, t1, y0, y1, y2;    // split planes between the passes (4 bytes per bin each)
     const __bf16 *tm() const { return tab_m.as<__bf16>(); }
     const __bf16 *tn() const { return m == n ? tab_m.as<__bf16>() : tab_n.as<__bf16>(); }
 };
 
 bool mfma_supported(int m, int n) {
-    return m >= 2 * kBM && n >= 2 * kBN * 2 && m % (2 * kBM) == 0 && n % (2 * kBN) == 0;
+    return m >= 2 * kTile && n >= 2 * kTile && m % (2 * kTile) == 0 && n % (2 * kTile) == 0;
 }
 
-int mfma_energy_partials(int m, int n) { return (m / kBM) * ((n / 2) / kBN); }
+// the last inverse pass writes one energy partial per tile of the potential
+int mfma_energy_partials(int m, int n) { return (m / kTile) * ((n / 2) / kTile); }
 
 // k_finalize runs one workgroup per four rows and writes one overflow partial each.
 int mfma_overflow_partials(int m) { return m / 4; }
@@ -468,8 +595,8 @@ int mfma_plan_create(MfmaPlan **out, int m, int n, hipStream_t s) {
     };
     make(p->tab_m, m);
     if (!rc && n != m) make(p->tab_n, n);
-    for (DevBuf *b : {&p->t0, &p->y0, &p->y1, &p->y2})
-        if (!rc) rc = b->alloc(sizeof(float) * B);
+    for (DevBuf *b : {&p->t0, &p->t1, &p->y0, &p->y1, &p->y2})
+        if (!rc) rc = b->alloc(sizeof(float) * B);    // four bf16 planes of B/2 each
     if (rc) {
         delete p;
         return rc;
@@ -481,44 +608,52 @@ int mfma_plan_create(MfmaPlan **out, int m, int n, hipStream_t s) {
 void mfma_plan_destroy(MfmaPlan *plan) { delete plan; }
 
 size_t mfma_plan_bytes(const MfmaPlan *p) {
-    return p ? p->tab_m.bytes + p->tab_n.bytes + p->t0.bytes + p->y0.bytes + p->y1.bytes + p->y2.bytes : 0;
+    return p ? p->tab_m.bytes + p->tab_n.bytes + p->t0.bytes + p->t1.bytes + p->y0.bytes +
+                   p->y1.bytes + p->y2.bytes
+             : 0;
 }
 
 namespace {
 
+// Split planes over a buffer of m*n floats (four planes of `rows` x `ld` bf16,
+// rows * ld = m*n/2).
+Split split_of(void *buf, int rows, int ld) { return Split{reinterpret_cast<__bf16 *>(buf), rows, ld}; }
+
 template <int MODE>
-int launch_mode(const MfmaArgs &a, int nz, hipStream_t s) {
+int launch_mode(const GemmArgs &a, int nz, hipStream_t s) {
     const int rows = (MODE == FWD_ROW || MODE == INV_ROW) ? a.m : a.m / 2;
     const int cols = (MODE == FWD_ROW || MODE == INV_ROW) ? a.n / 2 : a.n;
-    const unsigned nb = (unsigned)((rows / kBM) * (cols / kBN));
-    hipLaunchKernelGGL(k_mfma<MODE>, dim3(nb, nz), dim3(256), 0, s, a);
+    const dim3 grid((unsigned)((rows / kTile) * (cols / kTile)),
+                    (MODE == FWD_ROW || MODE == FWD_COL) ? 2u : (unsigned)nz);
+    switch (a.H / kKb) {   // H = L/2 with L a power of two in [128, 2048]
+    case 2: hipLaunchKernelGGL((k_gemm<MODE, 2>), grid, dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_gemm<MODE, 4>), grid, dim3(256), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_gemm<MODE, 8>), grid, dim3(256), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((k_gemm<MODE, 16>), grid, dim3(256), 0, s, a); break;
+    case 32: hipLaunchKernelGGL((k_gemm<MODE, 32>), grid, dim3(256), 0, s, a); break;
+    default: return fail(ERP_ERR_UNSUPPORTED, "MFMA transform length must be a power of two in [128, 2048]");
+    }
     ERP_HIP_TRY(hipGetLastError());
     return ERP_OK;
 }
 
-MfmaArgs base_args(const MfmaPlan *p, int m, int n) {
-    MfmaArgs a{};
-    a.m = m;
-    a.n = n;
-    (void)p;
-    return a;
-}
-
-// rows of x (m x n, natural) -> de-interleaved row transform T; then columns.
-int forward(const MfmaPlan *p, const float *x, float *T, const MfmaArgs &colargs, hipStream_t s,
-            KernelTimer *t) {
-    MfmaArgs a = base_args(p, p->m, p->n);
-    a.src[0] = x;
-    a.dst[0] = T;
-    a.tab = p->tn();
+// row-pass operand -> column-pass operand -> (spectra | natural a)
+int forward(const MfmaPlan *p, Split rowop, Split colop, GemmArgs c, hipStream_t s, KernelTimer *t) {
+    GemmArgs a{};
+    a.m = p->m;
+    a.n = p->n;
+    a.in[0] = rowop;
+    a.out[0] = colop;
+    a.op = p->tn();
     a.H = p->n / 2;
     if (t) t->begin("mfma_fwd_row");
     int rc = launch_mode<FWD_ROW>(a, 1, s);
     if (t) t->end();
     if (rc) return rc;
-    MfmaArgs c = colargs;
-    c.src[0] = T;
-    c.tab = p->tm();
+    c.m = p->m;
+    c.n = p->n;
+    c.in[0] = colop;
+    c.op = p->tm();
     c.H = p->m / 2;
     if (t) t->begin("mfma_fwd_col");
     rc = launch_mode<FWD_COL>(c, 1, s);
@@ -526,22 +661,30 @@ int forward(const MfmaPlan *p, const float *x, float *T, const MfmaArgs &colargs
     return rc;
 }
 
-// de-interleaved spectra X[z] -> column inverses Y[z] -> row inverses out[z].
-int inverse(const MfmaPlan *p, const float *const *X, float *const *Y, float *const *out,
+// split spectra X[z] -> column inverses Y[z] -> row inverses out[z].
+int inverse(const MfmaPlan *p, const Split *X, const Split *Y, float *const *out,
             const int *out_stride, int nz, int col_sine, int row_sine, const float *rho,
             double *energy_part, hipStream_t s, KernelTimer *t) {
-    MfmaArgs a = base_args(p, p->m, p->n);
-    for (int z = 0; z < nz; ++z) { a.src[z] = X[z]; a.dst[z] = Y[z]; }
-    a.tab = p->tm();
+    GemmArgs a{};
+    a.m = p->m;
+    a.n = p->n;
+    for (int z = 0; z < nz; ++z) { a.in[z] = X[z]; a.out[z] = Y[z]; }
+    a.op = p->tm();
     a.H = p->m / 2;
     a.sine_mask = col_sine;
     if (t) t->begin("mfma_inv_col");
     int rc = launch_mode<INV_COL>(a, nz, s);
     if (t) t->end();
     if (rc) return rc;
-    MfmaArgs b = base_args(p, p->m, p->n);
-    for (int z = 0; z < nz; ++z) { b.src[z] = Y[z]; b.dst[z] = out[z]; b.dst_stride[z] = out_stride ? out_stride[z] : 1; }
-    b.tab = p->tn();
+    GemmArgs b{};
+    b.m = p->m;
+    b.n = p->n;
+    for (int z = 0; z < nz; ++z) {
+        b.in[z] = Y[z];
+        b.dst[z] = out[z];
+        b.dst_stride[z] = out_stride ? out_stride[z] : 1;
+    }
+    b.op = p->tn();
     b.H = p->n / 2;
     b.sine_mask = row_sine;
     b.rho = rho;
@@ -558,24 +701,25 @@ int inverse(const MfmaPlan *p, const float *const *X, float *const *Y, float *co
 int launch_potential_mfma(const PotentialBufs &b, MfmaPlan *p, hipStream_t s, KernelTimer *t) {
     if (!p || p->m != b.m || p->n != b.n) return fail(ERP_ERR_INVALID, "MFMA plan does not match the grid");
     const int m = b.m, n = b.n;
+    const Split rowop = split_of(b.rowt, m, n / 2);         // [half][hl][row][k < n/2]
+    const Split colop = split_of(p->t0.p, n, m / 2);        // [half][hl][column][k < m/2]
     if (t) t->begin("mfma_finalize");
-    hipLaunchKernelGGL(k_finalize, dim3((unsigned)mfma_overflow_partials(m)), dim3(1024), 0, s, b);
+    hipLaunchKernelGGL(k_finalize, dim3((unsigned)mfma_overflow_partials(m)), dim3(1024),
+                       sizeof(float) * 4 * (size_t)n, s, b, rowop);
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
-    MfmaArgs c = base_args(p, m, n);
-    c.dst[0] = b.p1;
-    c.dst[1] = b.p2;
-    c.dst[2] = b.p3;
+    GemmArgs c{};
+    const Split X[3] = {split_of(b.p1, n, m / 2), split_of(b.p2, n, m / 2), split_of(b.p3, n, m / 2)};
+    for (int z = 0; z < 3; ++z) c.out[z] = X[z];
     c.spectral = 1;
     c.scale = (float)(4.0 / ((double)m * (double)n));
     c.two_pi_over_m = (float)(2.0 * M_PI) / (float)m;
     c.two_pi_over_n = (float)(2.0 * M_PI) / (float)n;
     c.wv_div = b.bw / b.bh;
     c.auv = b.auv;
-    int rc = forward(p, b.rho, b.rowt, c, s, t);
+    int rc = forward(p, rowop, colop, c, s, t);
     if (rc) return rc;
-    const float *X[3] = {b.p1, b.p2, b.p3};
-    float *Y[3] = {p->y0.as<float>(), p->y1.as<float>(), p->y2.as<float>()};
+    const Split Y[3] = {split_of(p->y0.p, m, n / 2), split_of(p->y1.p, m, n / 2), split_of(p->y2.p, m, n / 2)};
     float *fxy = reinterpret_cast<float *>(b.fxy);
     float *out[3] = {fxy, fxy + 1, b.pot};
     const int stride[3] = {2, 2, 1};
@@ -587,11 +731,16 @@ int launch_potential_mfma(const PotentialBufs &b, MfmaPlan *p, hipStream_t s, Ke
 
 // Leaf transforms (host-facing entry points run these on device buffers).
 int launch_dct2_mfma(const float *x, float *out, MfmaPlan *p, hipStream_t s) {
-    MfmaArgs c = base_args(p, p->m, p->n);
+    const int m = p->m, n = p->n;
+    const Split rowop = split_of(p->t1.p, m, n / 2);
+    const Split colop = split_of(p->t0.p, n, m / 2);
+    hipLaunchKernelGGL(k_prep_rows, dim3(cdiv((int64_t)m * (n / 16), 256)), dim3(256), 0, s, x, rowop, m, n);
+    ERP_HIP_TRY(hipGetLastError());
+    GemmArgs c{};
     c.dst[0] = out;
     c.spectral = 0;
-    c.scale = (float)(4.0 / ((double)p->m * (double)p->n));
-    return forward(p, x, p->t0.as<float>(), c, s, nullptr);
+    c.scale = (float)(4.0 / ((double)m * (double)n));
+    return forward(p, rowop, colop, c, s, nullptr);
 }
 
 // which: 0 idct2 (cos, cos), 1 idct_idxst (cos along x, sin along y),
@@ -599,11 +748,11 @@ int launch_dct2_mfma(const float *x, float *out, MfmaPlan *p, hipStream_t s) {
 int launch_inverse_mfma(const float *x, float *out, int which, MfmaPlan *p, hipStream_t s) {
     const int m = p->m, n = p->n;
     const int eps_u = which != 2, eps_v = which != 1;
-    hipLaunchKernelGGL(k_deinterleave, dim3(cdiv((int64_t)m * n, 256)), dim3(256), 0, s, x,
-                       p->t0.as<float>(), m, n, eps_u, eps_v);
+    const Split X[1] = {split_of(p->t0.p, n, m / 2)};
+    hipLaunchKernelGGL(k_prep_spectrum, dim3(cdiv((int64_t)n * 2 * (m / 16), 256)), dim3(256), 0, s, x,
+                       X[0], m, n, eps_u, eps_v);
     ERP_HIP_TRY(hipGetLastError());
-    const float *X[1] = {p->t0.as<float>()};
-    float *Y[1] = {p->y0.as<float>()};
+    const Split Y[1] = {split_of(p->y0.p, m, n / 2)};
     float *O[1] = {out};
     return inverse(p, X, Y, O, nullptr, 1, which == 2 ? 1 : 0, which == 1 ? 1 : 0, nullptr, nullptr, s,
                    nullptr);
