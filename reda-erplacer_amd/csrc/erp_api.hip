@@ -317,10 +317,8 @@ size_t erp::estimate_ctx_bytes(const erp_design *d, const erp_config *cfg) {
     const size_t per_bin = 2 * 8 + 9 * 4;                             // q_mov/q_fix + float maps
     const size_t partials = 16 * (P / 64 + C / 128 + (size_t)cfg->num_bins_x) + 64 * 1024;
     size_t mfma = 0;
-    if (cfg->spectral_mode == ERP_SPECTRAL_MFMA_BF16X3) {   // operator planes + 5 split-plane maps
-        const size_t hm = (size_t)cfg->num_bins_x / 2, hn = (size_t)cfg->num_bins_y / 2;
-        mfma = 24 * (hm * hm + (hm == hn ? 0 : hn * hn)) + 20 * B + 8 * B / 4096 * 4;
-    }
+    if (cfg->spectral_mode == ERP_SPECTRAL_MFMA_BF16X3)     // operator planes + 8 split maps
+        mfma = mfma_plan_estimate((int)cfg->num_bins_x, (int)cfg->num_bins_y) + 8 * B / 4096 * 4;
     return N * per_node + C * per_cell + P * per_pin + E * 4 + B * per_bin + partials + mfma +
            2 * 8 * (size_t)(cfg->num_bins_x + cfg->num_bins_y) + (1u << 20);
 }

@@ -193,6 +193,7 @@ bool mfma_supported(int m, int n);             // at least 128 bins per axis
 int mfma_energy_partials(int m, int n);        // energy partials the MFMA pipeline writes
 int mfma_overflow_partials(int m);             // overflow partials the MFMA pipeline writes
 size_t mfma_plan_bytes(const MfmaPlan *plan);
+size_t mfma_plan_estimate(int m, int n);       // device bytes a plan for an m x n grid takes
 // leaf transforms on device buffers (which: 0 idct2, 1 idct_idxst, 2 idxst_idct)
 int launch_dct2_mfma(const float *x, float *out, MfmaPlan *plan, hipStream_t s);
 int launch_inverse_mfma(const float *x, float *out, int which, MfmaPlan *plan, hipStream_t s);

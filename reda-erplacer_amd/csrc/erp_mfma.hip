@@ -53,8 +53,14 @@ constexpr int kTile = 64;               // output rows and columns of a workgrou
 constexpr int kKb = 32;                 // K per stage: one 16x16x32 MFMA step
 constexpr int kPlane = kTile * kKb;     // bf16 per LDS operand tile (4 KB)
 constexpr int kTp = kTile + 1;          // padded row of the f32 epilogue tiles
+// Row pitch of every bf16 plane: K plus 64 (128 bytes), so that the 16 rows one
+// fragment load touches do not all fall into the same few L2 channels (a
+// power-of-two pitch maps them onto a quarter of the channels or fewer).
+constexpr int kPad = 64;
+__host__ __device__ constexpr int pitch_of(int k) { return k + kPad; }
 
-// Four planes of one split map: [half][hi/lo], each `rows` x `ld` bf16, K contiguous.
+// Four planes of one split map: [half][hi/lo], each `rows` x `ld` bf16, K
+// contiguous (K < ld: the last kPad of each row are padding).
 struct Split {
     __bf16 *base;
     int rows, ld;
@@ -66,6 +72,7 @@ __host__ __device__ inline __bf16 *plane(const Split &s, int half, int hl) {
 struct GemmArgs {
     const __bf16 *op;         // operator planes of the transformed length
     int H;                    // half length: the K of every product
+    int ldo;                  // row pitch of the operator planes
     Split in[3];              // data operand per transform z
     Split out[3];             // FWD_ROW / FWD_COL / INV_COL: split result per z
     float *dst[3];            // INV_ROW result per z; FWD_COL without spectra: natural a
@@ -87,7 +94,7 @@ __device__ __forceinline__ T pick3(const T (&v)[3], int z) {
 }
 
 __device__ __forceinline__ const __bf16 *op_plane(const GemmArgs &a, int t, int hl) {
-    return a.op + (size_t)(2 * t + hl) * a.H * a.H;
+    return a.op + (size_t)(2 * t + hl) * a.H * a.ldo;
 }
 
 __device__ __forceinline__ void split8(const float *v, bf16x8 &hi, bf16x8 &lo) {
@@ -108,9 +115,20 @@ __device__ __forceinline__ int swz(int row, int chunk) {
 
 // XCD-aware tile order: consecutive workgroups go round-robin over the 8 XCDs;
 // give each XCD a contiguous run of tiles (tiles of one row band share data).
-__device__ __forceinline__ int tile_of_block(int b, int nb) {
-    return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+__device__ __forceinline__ int tile_of_block(int b, int nb, int nct) {
+    if (nb & 7) return b;
+    const int per = nb >> 3, nrt = nb / nct;
+    int bc = 1;
+    while (bc * bc < per) bc <<= 1;           // 2^ceil(p/2) for per = 2^p
+    bc = bc > nct ? nct : bc;
+    int br = per / bc;
+    if (br > nrt) { br = nrt; bc = per / nrt; }
+    const int x = b & 7, l = b >> 3;
+    const int bpr = nct / bc;                 // XCD blocks per block row
+    return ((x / bpr) * br + l / bc) * nct + (x % bpr) * bc + l % bc;
 }
+
+
 
 template <int MODE>
 struct GemmShape {
@@ -227,7 +245,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 
     const int out_cols = (MODE == FWD_ROW || MODE == INV_ROW) ? n / 2 : n;
     const int nct = out_cols / kTile;
-    const int tile = tile_of_block(blockIdx.x, gridDim.x);
+    const int tile = tile_of_block(blockIdx.x, gridDim.x, nct);
     const int rt = tile / nct, ct = tile % nct;
     const int r0 = rt * kTile, c0 = ct * kTile;
 
@@ -247,9 +265,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
             const __bf16 *pa, *pb;
             if (MODE == FWD_ROW || MODE == INV_ROW) {            // A = data rows, B = operator
                 pa = plane(d, half, hl) + (size_t)arow * d.ld;
-                pb = op_plane(a, te + half, hl) + (size_t)brow * a.H;
+                pb = op_plane(a, te + half, hl) + (size_t)brow * a.ldo;
             } else {                                             // A = operator, B = data columns
-                pa = op_plane(a, te + half, hl) + (size_t)arow * a.H;
+                pa = op_plane(a, te + half, hl) + (size_t)arow * a.ldo;
                 pb = plane(d, half, hl) + (size_t)brow * d.ld;
             }
             src[2 * h + hl] = pa + 8 * lc;
@@ -534,9 +552,9 @@ __global__ void k_prep_spectrum(const float *__restrict__ x, Split cols, int m, 
     *reinterpret_cast<bf16x8 *>(plane(cols, q, 1) + at) = lo;
 }
 
-// Operator planes of length L in FP64, split to bf16 pairs.
+// Operator planes of length L in FP64, split to bf16 pairs (row pitch pitch_of(H)).
 __global__ void k_make_planes(__bf16 *tab, int L) {
-    const int H = L / 2;
+    const int H = L / 2, ld = pitch_of(H);
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (int64_t)T_COUNT * H * H) return;
     const int t = (int)(idx / ((int64_t)H * H));
@@ -553,21 +571,34 @@ __global__ void k_make_planes(__bf16 *tab, int L) {
     const float f = (float)v;
     const __bf16 hi = (__bf16)f;
     const __bf16 lo = (__bf16)(float)(v - (double)(float)hi);
-    tab[(size_t)(2 * t) * H * H + rem] = hi;
-    tab[(size_t)(2 * t + 1) * H * H + rem] = lo;
+    tab[(size_t)(2 * t) * H * ld + (size_t)r * ld + c] = hi;
+    tab[(size_t)(2 * t + 1) * H * ld + (size_t)r * ld + c] = lo;
 }
 
 inline unsigned cdiv(int64_t a, int b) { return (unsigned)((a + b - 1) / b); }
 
 }  // namespace
 
+// Split maps between the passes: the row pass operand (m rows, K = n/2), the
+// column pass operand and the spectra (n rows, K = m/2), the column inverses
+// (m rows, K = n/2).
 struct MfmaPlan {
     int m = 0, n = 0;
     DevBuf tab_m, tab_n;          // operator planes (tab_n unused when m == n)
-    DevBuf t0, t1, y0, y1, y2;    // split planes between the passes (4 bytes per bin each)
+    DevBuf row, col, x[3], y[3];
     const __bf16 *tm() const { return tab_m.as<__bf16>(); }
     const __bf16 *tn() const { return m == n ? tab_m.as<__bf16>() : tab_n.as<__bf16>(); }
 };
+
+namespace {
+size_t table_bytes(int L) { return sizeof(__bf16) * 2 * T_COUNT * (size_t)(L / 2) * pitch_of(L / 2); }
+size_t split_bytes(int rows, int k) { return sizeof(__bf16) * 4 * (size_t)rows * pitch_of(k); }
+}  // namespace
+
+size_t mfma_plan_estimate(int m, int n) {
+    return table_bytes(m) + (m == n ? 0 : table_bytes(n)) + 4 * split_bytes(m, n / 2) +
+           4 * split_bytes(n, m / 2);
+}
 
 bool mfma_supported(int m, int n) {
     return m >= 2 * kTile && n >= 2 * kTile && m % (2 * kTile) == 0 && n % (2 * kTile) == 0;
@@ -585,18 +616,19 @@ int mfma_plan_create(MfmaPlan **out, int m, int n, hipStream_t s) {
     MfmaPlan *p = new MfmaPlan();
     p->m = m;
     p->n = n;
-    const size_t B = (size_t)m * n;
     int rc = ERP_OK;
     auto make = [&](DevBuf &b, int L) {
         const int64_t cnt = (int64_t)T_COUNT * (L / 2) * (L / 2);
-        if ((rc = b.alloc(sizeof(__bf16) * 2 * (size_t)cnt))) return;
+        if ((rc = b.alloc(table_bytes(L)))) return;
         hipLaunchKernelGGL(k_make_planes, dim3(cdiv(cnt, 256)), dim3(256), 0, s, b.as<__bf16>(), L);
         if (hipGetLastError() != hipSuccess) rc = fail(ERP_ERR_HIP, "k_make_planes launch failed");
     };
     make(p->tab_m, m);
     if (!rc && n != m) make(p->tab_n, n);
-    for (DevBuf *b : {&p->t0, &p->t1, &p->y0, &p->y1, &p->y2})
-        if (!rc) rc = b->alloc(sizeof(float) * B);    // four bf16 planes of B/2 each
+    for (DevBuf *b : {&p->row, &p->y[0], &p->y[1], &p->y[2]})
+        if (!rc) rc = b->alloc(split_bytes(m, n / 2));
+    for (DevBuf *b : {&p->col, &p->x[0], &p->x[1], &p->x[2]})
+        if (!rc) rc = b->alloc(split_bytes(n, m / 2));
     if (rc) {
         delete p;
         return rc;
@@ -608,16 +640,18 @@ int mfma_plan_create(MfmaPlan **out, int m, int n, hipStream_t s) {
 void mfma_plan_destroy(MfmaPlan *plan) { delete plan; }
 
 size_t mfma_plan_bytes(const MfmaPlan *p) {
-    return p ? p->tab_m.bytes + p->tab_n.bytes + p->t0.bytes + p->t1.bytes + p->y0.bytes +
-                   p->y1.bytes + p->y2.bytes
-             : 0;
+    if (!p) return 0;
+    size_t t = p->tab_m.bytes + p->tab_n.bytes + p->row.bytes + p->col.bytes;
+    for (int z = 0; z < 3; ++z) t += p->x[z].bytes + p->y[z].bytes;
+    return t;
 }
 
 namespace {
 
-// Split planes over a buffer of m*n floats (four planes of `rows` x `ld` bf16,
-// rows * ld = m*n/2).
-Split split_of(void *buf, int rows, int ld) { return Split{reinterpret_cast<__bf16 *>(buf), rows, ld}; }
+// Split planes of `rows` rows and K = k over a plan buffer (row pitch pitch_of(k)).
+Split split_of(const DevBuf &buf, int rows, int k) {
+    return Split{reinterpret_cast<__bf16 *>(buf.p), rows, pitch_of(k)};
+}
 
 template <int MODE>
 int launch_mode(const GemmArgs &a, int nz, hipStream_t s) {
@@ -646,6 +680,7 @@ int forward(const MfmaPlan *p, Split rowop, Split colop, GemmArgs c, hipStream_t
     a.out[0] = colop;
     a.op = p->tn();
     a.H = p->n / 2;
+    a.ldo = pitch_of(a.H);
     if (t) t->begin("mfma_fwd_row");
     int rc = launch_mode<FWD_ROW>(a, 1, s);
     if (t) t->end();
@@ -655,6 +690,7 @@ int forward(const MfmaPlan *p, Split rowop, Split colop, GemmArgs c, hipStream_t
     c.in[0] = colop;
     c.op = p->tm();
     c.H = p->m / 2;
+    c.ldo = pitch_of(c.H);
     if (t) t->begin("mfma_fwd_col");
     rc = launch_mode<FWD_COL>(c, 1, s);
     if (t) t->end();
@@ -671,6 +707,7 @@ int inverse(const MfmaPlan *p, const Split *X, const Split *Y, float *const *out
     for (int z = 0; z < nz; ++z) { a.in[z] = X[z]; a.out[z] = Y[z]; }
     a.op = p->tm();
     a.H = p->m / 2;
+    a.ldo = pitch_of(a.H);
     a.sine_mask = col_sine;
     if (t) t->begin("mfma_inv_col");
     int rc = launch_mode<INV_COL>(a, nz, s);
@@ -686,6 +723,7 @@ int inverse(const MfmaPlan *p, const Split *X, const Split *Y, float *const *out
     }
     b.op = p->tn();
     b.H = p->n / 2;
+    b.ldo = pitch_of(b.H);
     b.sine_mask = row_sine;
     b.rho = rho;
     b.energy_part = energy_part;
@@ -701,15 +739,15 @@ int inverse(const MfmaPlan *p, const Split *X, const Split *Y, float *const *out
 int launch_potential_mfma(const PotentialBufs &b, MfmaPlan *p, hipStream_t s, KernelTimer *t) {
     if (!p || p->m != b.m || p->n != b.n) return fail(ERP_ERR_INVALID, "MFMA plan does not match the grid");
     const int m = b.m, n = b.n;
-    const Split rowop = split_of(b.rowt, m, n / 2);         // [half][hl][row][k < n/2]
-    const Split colop = split_of(p->t0.p, n, m / 2);        // [half][hl][column][k < m/2]
+    const Split rowop = split_of(p->row, m, n / 2);         // [half][hl][row][k < n/2]
+    const Split colop = split_of(p->col, n, m / 2);         // [half][hl][column][k < m/2]
     if (t) t->begin("mfma_finalize");
     hipLaunchKernelGGL(k_finalize, dim3((unsigned)mfma_overflow_partials(m)), dim3(1024),
                        sizeof(float) * 4 * (size_t)n, s, b, rowop);
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
     GemmArgs c{};
-    const Split X[3] = {split_of(b.p1, n, m / 2), split_of(b.p2, n, m / 2), split_of(b.p3, n, m / 2)};
+    const Split X[3] = {split_of(p->x[0], n, m / 2), split_of(p->x[1], n, m / 2), split_of(p->x[2], n, m / 2)};
     for (int z = 0; z < 3; ++z) c.out[z] = X[z];
     c.spectral = 1;
     c.scale = (float)(4.0 / ((double)m * (double)n));
@@ -719,7 +757,7 @@ int launch_potential_mfma(const PotentialBufs &b, MfmaPlan *p, hipStream_t s, Ke
     c.auv = b.auv;
     int rc = forward(p, rowop, colop, c, s, t);
     if (rc) return rc;
-    const Split Y[3] = {split_of(p->y0.p, m, n / 2), split_of(p->y1.p, m, n / 2), split_of(p->y2.p, m, n / 2)};
+    const Split Y[3] = {split_of(p->y[0], m, n / 2), split_of(p->y[1], m, n / 2), split_of(p->y[2], m, n / 2)};
     float *fxy = reinterpret_cast<float *>(b.fxy);
     float *out[3] = {fxy, fxy + 1, b.pot};
     const int stride[3] = {2, 2, 1};
@@ -732,8 +770,8 @@ int launch_potential_mfma(const PotentialBufs &b, MfmaPlan *p, hipStream_t s, Ke
 // Leaf transforms (host-facing entry points run these on device buffers).
 int launch_dct2_mfma(const float *x, float *out, MfmaPlan *p, hipStream_t s) {
     const int m = p->m, n = p->n;
-    const Split rowop = split_of(p->t1.p, m, n / 2);
-    const Split colop = split_of(p->t0.p, n, m / 2);
+    const Split rowop = split_of(p->row, m, n / 2);
+    const Split colop = split_of(p->col, n, m / 2);
     hipLaunchKernelGGL(k_prep_rows, dim3(cdiv((int64_t)m * (n / 16), 256)), dim3(256), 0, s, x, rowop, m, n);
     ERP_HIP_TRY(hipGetLastError());
     GemmArgs c{};
@@ -748,11 +786,11 @@ int launch_dct2_mfma(const float *x, float *out, MfmaPlan *p, hipStream_t s) {
 int launch_inverse_mfma(const float *x, float *out, int which, MfmaPlan *p, hipStream_t s) {
     const int m = p->m, n = p->n;
     const int eps_u = which != 2, eps_v = which != 1;
-    const Split X[1] = {split_of(p->t0.p, n, m / 2)};
+    const Split X[1] = {split_of(p->x[0], n, m / 2)};
     hipLaunchKernelGGL(k_prep_spectrum, dim3(cdiv((int64_t)n * 2 * (m / 16), 256)), dim3(256), 0, s, x,
                        X[0], m, n, eps_u, eps_v);
     ERP_HIP_TRY(hipGetLastError());
-    const Split Y[1] = {split_of(p->y0.p, m, n / 2)};
+    const Split Y[1] = {split_of(p->y[0], m, n / 2)};
     float *O[1] = {out};
     return inverse(p, X, Y, O, nullptr, 1, which == 2 ? 1 : 0, which == 1 ? 1 : 0, nullptr, nullptr, s,
                    nullptr);
