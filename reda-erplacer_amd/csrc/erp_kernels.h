@@ -5,6 +5,24 @@
 
 namespace erp {
 
+// Loads through the global address space.  A job's buffers are reached through
+// pointers read from its descriptor in memory, which the compiler cannot place
+// in an address space: it emits flat loads, and a flat load counts against the
+// same wait counter as the scalar loads of those pointers, so every pointer load
+// then waits for all vector loads in flight.
+// gld(p) loads *p (a type of 4, 8 or 16 bytes) through a global pointer to an
+// integer vector of the same size and reinterprets the bits.
+#define ERP_GLOBAL __attribute__((address_space(1)))
+template <int B> struct GldWord;
+template <> struct GldWord<4> { typedef unsigned type; };
+template <> struct GldWord<8> { typedef unsigned type __attribute__((ext_vector_type(2))); };
+template <> struct GldWord<16> { typedef unsigned type __attribute__((ext_vector_type(4))); };
+template <class T>
+__device__ __forceinline__ T gld(const T *p) {
+    typedef typename GldWord<sizeof(T)>::type W;
+    return __builtin_bit_cast(T, *(const ERP_GLOBAL W *)p);
+}
+
 // Workgroups are handed to the eight XCDs round robin (launch index w runs on
 // XCD w mod 8), and each XCD has its own L2.  xcd_block(w, nb) renumbers the nb
 // workgroups of one job so that every XCD processes one contiguous range of

@@ -31,8 +31,8 @@ __device__ __forceinline__ float tri_unclamped(float x, float s, int k, float bs
 // bins [max(trunc(x/bw), 0), min(trunc((x+s)/bw) + 1, M)).
 __device__ __forceinline__ float2 efield_cell(const CellBufs &c, int64_t i, float inv_bw,
                                               float inv_bh) {
-    const float2 p = c.pos[i];
-    const float4 gq = c.cgeo[i];
+    const float2 p = gld(c.pos + i);
+    const float4 gq = gld(c.cgeo + i);
     const float2 o = make_float2(gq.x, gq.y), sz = make_float2(gq.z, gq.w);
     const float nx = p.x + o.x, ny = p.y + o.y;
     int xl = (int)((nx - 0.f) * inv_bw);
@@ -53,12 +53,12 @@ __device__ __forceinline__ float2 efield_cell(const CellBufs &c, int64_t i, floa
         for (int k = xl; k < xh; ++k) {
             const float px = tri_unclamped(nx, sz.x, k, c.bw);
             const float4 *b0 = reinterpret_cast<const float4 *>(c.fxy + fxy_index(k, hb, c.n));
-            const float4 v0 = b0[0], v1 = b0[1];
+            const float4 v0 = gld(b0), v1 = gld(b0 + 1);
             float4 v2 = make_float4(0.f, 0.f, 0.f, 0.f), v3 = v2;
             if (two) {
                 const float4 *b1 = reinterpret_cast<const float4 *>(c.fxy + fxy_index(k, hb + 4, c.n));
-                v2 = b1[0];
-                v3 = b1[1];
+                v2 = gld(b1);
+                v3 = gld(b1 + 1);
             }
             const float vx[8] = {v0.x, v0.z, v1.x, v1.z, v2.x, v2.z, v3.x, v3.z};
             const float vy[8] = {v0.y, v0.w, v1.y, v1.w, v2.y, v2.w, v3.y, v3.w};
@@ -76,29 +76,29 @@ __device__ __forceinline__ float2 efield_cell(const CellBufs &c, int64_t i, floa
         for (int k = xl; k < xh; ++k) {
             const float px = tri_unclamped(nx, sz.x, k, c.bw);
             for (int h = yl; h < yh; ++h) {
-                const float2 v = c.fxy[fxy_index(k, h, c.n)];
+                const float2 v = gld(c.fxy + fxy_index(k, h, c.n));
                 const float area = px * tri_unclamped(ny, sz.y, h, c.bh);
                 gx += area * v.x;
                 gy += area * v.y;
             }
         }
     }
-    const float r = c.cprop[i].x;
+    const float r = gld(c.cprop + i).x;
     return make_float2(-gx * r, -gy * r);
 }
 
 // hpwl.rs:120-134: sum of the cell's pin gradients in macro_2_pins order
 // (slot indices and gradients of four pins are loaded before they are summed).
 __device__ __forceinline__ float2 pin2cell(const CellBufs &c, int64_t i) {
-    const int k0 = c.cstart[i], k1 = c.cstart[i + 1];
+    const int k0 = gld(c.cstart + i), k1 = gld(c.cstart + i + 1);
     float gx = 0.f, gy = 0.f;
     for (int k = k0; k < k1; k += 4) {
         int sl[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) sl[j] = k + j < k1 ? c.cslots[k + j] : -1;
+        for (int j = 0; j < 4; ++j) sl[j] = k + j < k1 ? gld(c.cslots + k + j) : -1;
         float2 g[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) g[j] = sl[j] >= 0 ? c.slot_grad[sl[j]] : make_float2(0.f, 0.f);
+        for (int j = 0; j < 4; ++j) g[j] = sl[j] >= 0 ? gld(c.slot_grad + sl[j]) : make_float2(0.f, 0.f);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             if (sl[j] >= 0) {
@@ -113,7 +113,7 @@ __device__ __forceinline__ float2 pin2cell(const CellBufs &c, int64_t i) {
 // nesterov.rs:75-95: preconditioned combined gradient
 __device__ __forceinline__ float2 combine(const CellBufs &c, int64_t i, float2 gwl, float2 gd,
                                           float dw) {
-    const float4 pr = c.cprop[i];
+    const float4 pr = gld(c.cprop + i);
     const float inv_p = 1.f / fminf(pr.y + dw * pr.z, 1.f);
     return make_float2((gwl.x + dw * gd.x) * inv_p, (gwl.y + dw * gd.y) * inv_p);
 }
@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(kBlock) k_l1(const JobDesc *__restrict__ jd) {
 // g and the BB dot-product partials sum |dx|^2, sum |dg|^2 (nesterov.rs:87-93)
 __device__ __forceinline__ void bb_terms(const CellBufs &c, int64_t i, float2 g, double &a,
                                          double &b) {
-    const float2 p = c.pos[i], pp = c.prev_pos[i], pg = c.prev_g[i];
+    const float2 p = gld(c.pos + i), pp = gld(c.prev_pos + i), pg = gld(c.prev_g + i);
     const double dx = (double)(p.x - pp.x), dy = (double)(p.y - pp.y);
     const double gx = (double)(g.x - pg.x), gy = (double)(g.y - pg.y);
     a = dx * dx + dy * dy;
@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_fused(const JobDesc *__restrict
     const DevState *__restrict__ st = c.st;
     double a = 0.0, b = 0.0;
     if (i < c.C) {
-        const float2 gwl = READ_GWL ? c.gwl[i] : pin2cell(c, i);
+        const float2 gwl = READ_GWL ? gld(c.gwl + i) : pin2cell(c, i);
         const float2 gd = efield_cell(c, i, 1.f / c.bw, 1.f / c.bh);
         const float2 g = combine(c, i, gwl, gd, (float)st->density_weight);
         c.g[i] = g;
@@ -230,6 +230,7 @@ __global__ void __launch_bounds__(kBlock) k_update(const JobDesc *__restrict__ j
 // Deterministic sums of the partial arrays by one workgroup, then the FP64
 // Nesterov and scheduler updates (one workgroup per job).
 constexpr int kScalarThreads = 1024;
+constexpr int kScalarUnroll = 8;
 
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
@@ -251,8 +252,17 @@ __global__ void __launch_bounds__(kScalarThreads) k_scalar(const JobDesc *__rest
     __shared__ double tot[4];
     __shared__ int64_t toti;
     __shared__ double pw[3];
-    // every thread sums a fixed strided subset of each partial array; the loads
-    // of all arrays are issued together (one memory round trip, not five)
+    // the device state is staged in LDS: its words are loaded together with the
+    // partials, updated there by one thread and stored back together
+    constexpr int kStateWords = (int)(sizeof(DevState) / 8);
+    static_assert(sizeof(DevState) % 8 == 0, "DevState is copied as 8-byte words");
+    __shared__ unsigned long long ssw[kStateWords];
+    const unsigned long long *gsw = reinterpret_cast<const unsigned long long *>(a.st);
+    const unsigned long long sword = threadIdx.x < kStateWords ? gld(gsw + threadIdx.x) : 0ull;
+    // every thread sums a fixed strided subset of each partial array, in
+    // increasing index; the loads of kScalarUnroll strides of all arrays are
+    // issued before any is added (one memory round trip per kScalarUnroll
+    // strides, not one per stride)
     double h = 0, e = 0, pa = 0, pb = 0;
     int64_t o = 0;
     const int nh = (ops & SC_HPWL) ? a.n_hpwl : 0;
@@ -260,14 +270,26 @@ __global__ void __launch_bounds__(kScalarThreads) k_scalar(const JobDesc *__rest
     const int ne = (ops & SC_POT) ? a.n_energy : 0;
     const int nc = (ops & (SC_L1 | SC_NEST)) ? a.n_cell : 0;
     const int most = max(max(nh, no), max(ne, nc));
-    for (int k = threadIdx.x; k < most; k += kScalarThreads) {
-        const double vh = k < nh ? a.hpwl_part[k] : 0.0;
-        const int64_t vo = k < no ? a.ovf_part[k] : 0;
-        const double ve = k < ne ? a.energy_part[k] : 0.0;
-        const double va = k < nc ? a.part_a[k] : 0.0;
-        const double vb = k < nc ? a.part_b[k] : 0.0;
-        h += vh; o += vo; e += ve; pa += va; pb += vb;
+    const double *hp = a.hpwl_part, *ep = a.energy_part, *ap = a.part_a, *bp = a.part_b;
+    const int64_t *op = a.ovf_part;
+    for (int k0 = threadIdx.x; k0 < most; k0 += kScalarUnroll * kScalarThreads) {
+        double vh[kScalarUnroll], ve[kScalarUnroll], va[kScalarUnroll], vb[kScalarUnroll];
+        int64_t vo[kScalarUnroll];
+#pragma unroll
+        for (int u = 0; u < kScalarUnroll; ++u) {
+            const int k = k0 + u * kScalarThreads;
+            vh[u] = k < nh ? gld(hp + k) : 0.0;
+            vo[u] = k < no ? gld(op + k) : 0;
+            ve[u] = k < ne ? gld(ep + k) : 0.0;
+            va[u] = k < nc ? gld(ap + k) : 0.0;
+            vb[u] = k < nc ? gld(bp + k) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kScalarUnroll; ++u) {
+            h += vh[u]; o += vo[u]; e += ve[u]; pa += va[u]; pb += vb[u];
+        }
     }
+    if (threadIdx.x < kStateWords) ssw[threadIdx.x] = sword;
     // fixed-order reduction: lanes of a wavefront by butterfly, then the waves in order
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     h = wave_sum_f64(h); e = wave_sum_f64(e); pa = wave_sum_f64(pa); pb = wave_sum_f64(pb);
@@ -289,7 +311,7 @@ __global__ void __launch_bounds__(kScalarThreads) k_scalar(const JobDesc *__rest
     __syncthreads();
     const double hsum = tot[0], esum = tot[1], asum = tot[2], bsum = tot[3];
     const int64_t osum = toti;
-    DevState *sp = a.st;
+    DevState *sp = reinterpret_cast<DevState *>(ssw);
     // scheduler.rs:92-130 needs three powers; lane 0 of waves 1..3 evaluates them
     // concurrently from the values the update below will see.
     if ((ops & SC_SCHED) && lane == 0 && wave >= 1 && wave <= 3) {
@@ -306,54 +328,57 @@ __global__ void __launch_bounds__(kScalarThreads) k_scalar(const JobDesc *__rest
         }
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    DevState &s = *sp;
-    if (ops & SC_HPWL) s.hpwl = hsum;
-    if (ops & SC_POT) {
-        s.overflow = (double)osum * a.inv_scale_over_area;
-        s.energy = esum;
-    }
-    if (ops & SC_L1) {                       // nesterov.rs:47-55
-        s.l1_wl = asum;
-        s.l1_d = bsum;
-        s.density_weight = s.density_weight * (asum / bsum);
-    }
-    if (ops & SC_NEST) {
-        // update_momentum (nesterov.rs:148-158), then the BB step (:98-100)
-        const double a_prev = s.alpha;
-        const double na = (1.0 + sqrt(4.0 * a_prev * a_prev + 1.0)) / 2.0;
-        s.beta = (a_prev - 1.0) / na;
-        s.alpha = na;
-        s.num = asum;
-        s.den = bsum;
-        if (s.iteration > 0 && bsum > 0.0) s.alpha = sqrt(asum / bsum);
-        s.dw_used = s.density_weight;
-    }
-    if (ops & SC_SCHED) {                    // scheduler.rs:68-130
-        s.iteration += 1;
-        if (!s.hpwl_has_initial) { s.hpwl_has_initial = 1; s.hpwl_initial = s.hpwl; }
-        s.hpwl_prev = s.hpwl_cur;
-        s.hpwl_cur = s.hpwl;
-        if (!s.ovf_has_initial) { s.ovf_has_initial = 1; s.ovf_initial = s.overflow; }
-        s.ovf_prev = s.ovf_cur;
-        s.ovf_cur = s.overflow;
-        const double delta = s.hpwl_cur - s.hpwl_prev;
-        double power;
-        if (delta < 0.0) {
-            power = pw[0];
-            power = power > 0.98 ? power : 0.98;
-        } else {
-            power = pw[1];
-            power = power < 0.95 ? 0.95 : (power > 1.05 ? 1.05 : power);
+    if (threadIdx.x == 0) {
+        DevState &s = *sp;
+        if (ops & SC_HPWL) s.hpwl = hsum;
+        if (ops & SC_POT) {
+            s.overflow = (double)osum * a.inv_scale_over_area;
+            s.energy = esum;
         }
-        s.density_weight = s.density_weight * (1.05 * power);
-        double coef = pw[2];
-        coef = coef < 0.05 ? 0.05 : (coef > 10.0 ? 10.0 : coef);
-        s.gamma = s.base_gamma * coef;
-        s.should_stop = s.hpwl_prev != 0.0 && s.hpwl_cur != 0.0 && s.hpwl_prev > s.hpwl_cur * 1.5;
+        if (ops & SC_L1) {                       // nesterov.rs:47-55
+            s.l1_wl = asum;
+            s.l1_d = bsum;
+            s.density_weight = s.density_weight * (asum / bsum);
+        }
+        if (ops & SC_NEST) {
+            // update_momentum (nesterov.rs:148-158), then the BB step (:98-100)
+            const double a_prev = s.alpha;
+            const double na = (1.0 + sqrt(4.0 * a_prev * a_prev + 1.0)) / 2.0;
+            s.beta = (a_prev - 1.0) / na;
+            s.alpha = na;
+            s.num = asum;
+            s.den = bsum;
+            if (s.iteration > 0 && bsum > 0.0) s.alpha = sqrt(asum / bsum);
+            s.dw_used = s.density_weight;
+        }
+        if (ops & SC_SCHED) {                    // scheduler.rs:68-130
+            s.iteration += 1;
+            if (!s.hpwl_has_initial) { s.hpwl_has_initial = 1; s.hpwl_initial = s.hpwl; }
+            s.hpwl_prev = s.hpwl_cur;
+            s.hpwl_cur = s.hpwl;
+            if (!s.ovf_has_initial) { s.ovf_has_initial = 1; s.ovf_initial = s.overflow; }
+            s.ovf_prev = s.ovf_cur;
+            s.ovf_cur = s.overflow;
+            const double delta = s.hpwl_cur - s.hpwl_prev;
+            double power;
+            if (delta < 0.0) {
+                power = pw[0];
+                power = power > 0.98 ? power : 0.98;
+            } else {
+                power = pw[1];
+                power = power < 0.95 ? 0.95 : (power > 1.05 ? 1.05 : power);
+            }
+            s.density_weight = s.density_weight * (1.05 * power);
+            double coef = pw[2];
+            coef = coef < 0.05 ? 0.05 : (coef > 10.0 ? 10.0 : coef);
+            s.gamma = s.base_gamma * coef;
+            s.should_stop = s.hpwl_prev != 0.0 && s.hpwl_cur != 0.0 && s.hpwl_prev > s.hpwl_cur * 1.5;
+        }
+        const double chk = s.hpwl + s.overflow + s.energy + s.alpha + s.beta + s.density_weight + s.gamma;
+        s.nonfinite = isfinite(chk) ? 0 : 1;
     }
-    const double chk = s.hpwl + s.overflow + s.energy + s.alpha + s.beta + s.density_weight + s.gamma;
-    s.nonfinite = isfinite(chk) ? 0 : 1;
+    __syncthreads();
+    if (threadIdx.x < kStateWords) reinterpret_cast<unsigned long long *>(a.st)[threadIdx.x] = ssw[threadIdx.x];
 }
 
 // (hpwl, overflow, nonfinite) of every job of a batch into one array.
