@@ -59,6 +59,8 @@ def kernel_bytes(name, d, grid, cells_pins):
         "sort_place": C * 68,
         # read q_mov, q_fix (16), clear q_mov (8), write density (4) and row transform (4)
         "pass1_row_dct": B * 32,
+        # pass 1 and the WA net groups up to degree 256 in one launch
+        "pass1_wa": B * 32 + (P2 + int(narrow.sum())) * 28 + int(narrow.size) * 8,
         # read row transform (4), write three column-inverse maps (12)
         "pass2_col_spectral": B * 16,
         # read three maps (12) + density (4), write field x/y and potential (12)

@@ -174,15 +174,25 @@ int phase_scheduler(erp_ctx *ctx) {
 constexpr unsigned kSideBranchBlocks = 512;
 
 // One iteration (iteration >= 1) as a fused kernel sequence: what the graphs hold.
-// fork: (graph capture only) after the scatter, the WA kernel and the per-cell
+// FFT form: the WA kernel runs inside pass 1's launch (k_pass1_wa), whose
+// two workgroups per CU leave room for it, on the one stream.  MFMA form with
+// fork (graph capture only): after the scatter, the WA kernel and the per-cell
 // sum of its pin gradients (k_pin2cell) run on a second branch beside the
-// spectral passes -- which leave most of the GPU idle (two workgroups per CU)
-// -- and join before the cell kernel, which then reads the summed gradient.
-// Both branches only read the positions; the results are bit-identical.
+// spectral passes and join before the cell kernel, which then reads the summed
+// gradient.  WA only reads the positions; the results are bit-identical.
 int launch_fused_iteration(erp_ctx *ctx, bool fork = false) {
     KernelTimer *t = ctx->timing ? ctx->timer : nullptr;
     int rc;
-    if (fork) {
+    if (ctx->spectral_mode != ERP_SPECTRAL_MFMA_BF16X3) {
+        // FFT form (captured and eager alike): WA shares pass 1's launch on the
+        // one stream (no fork and join), and the cell kernel sums the pin
+        // gradients itself
+        if ((rc = run_scatter(ctx))) return rc;
+        rc = launch_potential_fft(djob(ctx), ctx->dims, 1, ctx->stream, t, /*with_wa=*/true);
+        if (ctx->keep_q) ctx->q_dirty = true;
+        if (rc) return rc;
+        fork = false;
+    } else if (fork) {
         if ((rc = run_scatter(ctx))) return rc;
         ERP_HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
         ERP_HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
@@ -274,10 +284,9 @@ int reset_ctx(erp_ctx *ctx) {
 
 int erp::launch_batched_iteration(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s) {
     const bool owner = bd.max_reach <= 2 && (double)bd.max_cells < 0.75 * bd.m * bd.n;
-    int rc = launch_wa(d, bd, nj, s);
-    if (!rc) rc = launch_sort(d, bd, nj, owner, s);
+    int rc = launch_sort(d, bd, nj, owner, s);
     if (!rc) rc = owner ? launch_scatter_owner(d, bd, nj, s) : launch_scatter(d, bd, nj, true, s);
-    if (!rc) rc = launch_potential_fft(d, bd, nj, s);
+    if (!rc) rc = launch_potential_fft(d, bd, nj, s, nullptr, /*with_wa=*/true);   // WA inside pass 1
     if (!rc) rc = launch_cell_fused(d, bd, nj, false, s);
     if (!rc) rc = launch_scalar(d, nj, SC_HPWL | SC_POT | SC_NEST | SC_SCHED, s);
     if (!rc) rc = launch_update(d, bd, nj, s);

@@ -100,6 +100,18 @@ struct WaNets {
     float2 *slot_grad;          // [P] output
     double *hpwl_part;          // partial sums, one per workgroup (see wa_partials)
 };
+// One launch for several net groups: the grid is cut into segments of
+// workgroups, each segment running one group (-1: two-pin nets, b >= 0: degree
+// bucket b), so small buckets overlap instead of each draining the GPU alone.
+struct WaPlan {
+    int nseg;
+    int bucket[kWaBuckets + 1];
+    unsigned begin[kWaBuckets + 2];
+};
+// The plans of a batch: net groups up to degree 256 (k_wa_narrow) and above.
+struct BatchDims;
+WaPlan wa_narrow_plan(const BatchDims &bd);
+WaPlan wa_wide_plan(const BatchDims &bd);
 // sets part_offset and returns the number of partials
 int wa_partials(WaNets &w);
 // Exact HPWL in int64 DBU over all nets.
@@ -187,6 +199,9 @@ struct BatchDims {
 // max_blocks > 0 caps the grid (the kernels loop over the plan's workgroups)
 int launch_wa(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
               KernelTimer *t = nullptr, unsigned max_blocks = 0);
+// the degree buckets above 256 only (k_wa_wide); launch_wa runs both
+int launch_wa_wide(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
+                   KernelTimer *t = nullptr, unsigned max_blocks = 0);
 int launch_sort(const JobDesc *d, const BatchDims &bd, int nj, bool owner_items, hipStream_t s,
                 KernelTimer *t = nullptr);   // owner_items: also build the owner-scatter work list
 // copy_perm: the sorted cell ids (aux.w) are copied back into perm, and the
@@ -207,7 +222,10 @@ int launch_update(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
 // out[3j..3j+2] = hpwl, overflow, nonfinite of job j
 int launch_gather_state(const JobDesc *d, int nj, double *out, hipStream_t s);
 int launch_scalar(const JobDesc *d, int nj, int ops, hipStream_t s);
+// with_wa: pass 1 shares its launch with the WA net groups up to degree 256
+// (the groups above run just before), their grid capped at wa_cap workgroups
+// when wa_cap > 0
 int launch_potential_fft(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
-                         KernelTimer *t = nullptr);
+                         KernelTimer *t = nullptr, bool with_wa = false, unsigned wa_cap = 0);
 
 }  // namespace erp

@@ -24,6 +24,7 @@
 #include "erp_internal.h"
 #include "erp_kernels.h"
 #include "erp_timer.h"
+#include "erp_wa.h"
 
 namespace erp {
 namespace {
@@ -137,7 +138,7 @@ __device__ __forceinline__ void row_dct_body(
     const float *__restrict__ x, int64_t *__restrict__ q_mov, const int64_t *__restrict__ q_fix,
     float *__restrict__ rho_out, int64_t *__restrict__ ovf_part, float *__restrict__ T, int n,
     int logN, int F, const float2 *__restrict__ fft_tw, const float2 *__restrict__ dct_tw,
-    double inv_scale, int64_t coef_q, float inv_bin_area, bool clear_q) {
+    double inv_scale, int64_t coef_q, float inv_bin_area, bool clear_q, int bx) {
     extern __shared__ float2 lds[];
     float2 *tw = lds;
     float2 *dtw_s = lds + (n >> 1);   // e^{-i pi k / 2L} staged in LDS
@@ -145,7 +146,7 @@ __device__ __forceinline__ void row_dct_body(
     load_tw(tw, fft_tw, n >> 1);
     load_tw(dtw_s, dct_tw, n);
     __syncthreads();
-    const int row_base = blockIdx.x * 2 * F;
+    const int row_base = bx * 2 * F;
     int64_t ovf = 0;
     // chunks of kCh elements per thread: every load of a chunk is issued
     // before any is used
@@ -201,7 +202,7 @@ __device__ __forceinline__ void row_dct_body(
             if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
             __syncthreads();
         }
-        if (threadIdx.x == 0) ovf_part[blockIdx.x] = red[0];
+        if (threadIdx.x == 0) ovf_part[bx] = red[0];
     }
     fft_stages<false>(buf, F, logN, tw);
     for (int idx = threadIdx.x; idx < (F << logN); idx += blockDim.x) {
@@ -219,7 +220,7 @@ __global__ void __launch_bounds__(kBlock) k_row_dct(const float *__restrict__ x,
                                                     const float2 *__restrict__ fft_tw,
                                                     const float2 *__restrict__ dct_tw) {
     row_dct_body<false>(x, nullptr, nullptr, nullptr, nullptr, T, n, logN, F, fft_tw, dct_tw, 0.0,
-                        0, 0.f, false);
+                        0, 0.f, false, (int)blockIdx.x);
 }
 
 // pass 1 of the potential pipeline for job blockIdx.y
@@ -227,7 +228,37 @@ __global__ void __launch_bounds__(kBlock) k_pass1(const JobDesc *__restrict__ jd
     if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
     const PotentialBufs &b = jd[blockIdx.y].pb;
     row_dct_body<true>(nullptr, b.q_mov, b.q_fix, b.rho, b.ovf_part, b.rowt, b.n, logN, F,
-                       b.tn_fft, b.tn_dct, b.inv_scale, b.coef_q, b.inv_bin_area, !b.keep_q);
+                       b.tn_fft, b.tn_dct, b.inv_scale, b.coef_q, b.inv_bin_area, !b.keep_q,
+                       (int)blockIdx.x);
+}
+
+// A range of the WA plan's workgroups run inside another launch (all of them
+// inside pass 1's: 4377 it/s at 1M cells against 4367 / 4345 / 4299 with 70 /
+// 50 / 30 % there and the rest inside pass 3's, whose 36 KB of LDS per
+// workgroup leave the WA workgroups little room).
+struct WaShare {
+    WaPlan plan;
+    unsigned first, last;   // plan workgroups [first, last)
+};
+
+// Pass 1 and part of the WA net groups up to degree 256 in one launch:
+// workgroups [0, npass) are pass-1 workgroups, the rest loop over their share
+// of the WA plan with their own stride.  The two are independent (WA reads the
+// positions, pass 1 the accumulators), each workgroup computes exactly what it
+// does in its own launch, and the launch needs no second stream and no events
+// to overlap them (pass 1's two workgroups per CU leave room).
+__global__ void __launch_bounds__(kBlock) k_pass1_wa(const JobDesc *__restrict__ jd, int logN, int F,
+                                                     int npass, WaShare wa) {
+    if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
+    if ((int)blockIdx.x < npass) {
+        const PotentialBufs &b = jd[blockIdx.y].pb;
+        row_dct_body<true>(nullptr, b.q_mov, b.q_fix, b.rho, b.ovf_part, b.rowt, b.n, logN, F,
+                           b.tn_fft, b.tn_dct, b.inv_scale, b.coef_q, b.inv_bin_area, !b.keep_q,
+                           (int)blockIdx.x);
+    } else {
+        wa_narrow_blocks(jd[blockIdx.y].wa, wa.plan, wa.first + (blockIdx.x - npass), gridDim.x - npass,
+                         wa.last);
+    }
 }
 
 // -------------------------------------------------------------- column DCT-II
@@ -567,6 +598,7 @@ __global__ void __launch_bounds__(kBlock) k_row_final(const JobDesc *__restrict_
     if (threadIdx.x == 0) energy_part[blockIdx.x] = red[0];
 }
 
+
 __global__ void k_make_tables(float2 *fft_tw, float2 *dct_tw, int L) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < L / 2) {
@@ -680,15 +712,29 @@ int launch_spectral_leaf(const float *x, float *out, float *scratch, int m, int 
 }
 
 int launch_potential_fft(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
-                         KernelTimer *t) {
+                         KernelTimer *t, bool with_wa, unsigned wa_cap) {
     const int m = bd.m, n = bd.n, F = kRowPairs, logN = ilog2(n), logM = ilog2(m), G = col_pairs(m);
-    // pass 1
+    // pass 1 (with the WA nets: the wide buckets first, on their own)
     size_t l1 = row_lds(n, F);
-    int rc = set_lds_limit((const void *)k_pass1, l1);
-    if (rc) return rc;
-    if (t) t->begin("pass1_row_dct");
-    hipLaunchKernelGGL(k_pass1, dim3(m / (2 * F), nj), dim3(kBlock), l1, s, d, logN, F);
-    if (t) t->end();
+    int rc;
+    const int npass = m / (2 * F);
+    if (with_wa) {
+        if ((rc = launch_wa_wide(d, bd, nj, s))) return rc;
+        const WaPlan plan = wa_narrow_plan(bd);
+        const unsigned total = plan.begin[plan.nseg];
+        const WaShare wa{plan, 0u, total};
+        const unsigned wa_grid = wa_cap ? std::min(total, wa_cap) : total;
+        if ((rc = set_lds_limit((const void *)k_pass1_wa, l1))) return rc;
+        if (t) t->begin("pass1_wa");
+        hipLaunchKernelGGL(k_pass1_wa, dim3((unsigned)npass + wa_grid, nj), dim3(kBlock), l1, s, d, logN, F,
+                           npass, wa);
+        if (t) t->end();
+    } else {
+        if ((rc = set_lds_limit((const void *)k_pass1, l1))) return rc;
+        if (t) t->begin("pass1_row_dct");
+        hipLaunchKernelGGL(k_pass1, dim3(npass, nj), dim3(kBlock), l1, s, d, logN, F);
+        if (t) t->end();
+    }
     ERP_HIP_TRY(hipGetLastError());
     // pass 2 (operator constants exactly as frequencies.rs computes them in T=f32)
     const float two_pi_over_m = (float)(2.0 * M_PI) / (float)m;
@@ -706,7 +752,7 @@ int launch_potential_fft(const JobDesc *d, const BatchDims &bd, int nj, hipStrea
     size_t l3 = row_lds(n, 3 * F);
     if ((rc = set_lds_limit((const void *)k_row_final, l3))) return rc;
     if (t) t->begin("pass3_row_final");
-    hipLaunchKernelGGL(k_row_final, dim3(m / (2 * F), nj), dim3(kBlock), l3, s, d, logN, F);
+    hipLaunchKernelGGL(k_row_final, dim3(npass, nj), dim3(kBlock), l3, s, d, logN, F);
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
     return ERP_OK;
