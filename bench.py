@@ -313,15 +313,29 @@ def end_state(workload, info, hpwl_dbu, trace=None):
             for it, h, _ in r["trace"]:
                 lo, hi = env.get(it, (h, h))
                 env[it] = (min(lo, h), max(hi, h))
+        # the oracle at the GPU's precision split (f32 arrays, FP64 scalars) and
+        # in plain f32 at the same iterations: while the one-ulp FP64 ensemble is
+        # still tight, the precisions have already parted
+        other = {}
+        for r in band["runs"]:
+            if r["sfx"] in ("_f32d", "_f32") and "trace" in r:
+                for it, h, _ in r["trace"]:
+                    other.setdefault((r["sfx"], it), []).append(h)
         rows = []
         for it, h, o in trace:
             if it in env:
                 lo, hi = env[it]
                 mid = 0.5 * (lo + hi)
-                rows.append({"iteration": it, "hpwl": h, "overflow": round(o, 5),
-                             "fp64_min": lo, "fp64_max": hi,
-                             "rel_to_fp64_mid": round(h / mid - 1, 5),
-                             "inside": bool(lo <= h <= hi)})
+                row = {"iteration": it, "hpwl": h, "overflow": round(o, 5),
+                       "fp64_min": lo, "fp64_max": hi,
+                       "rel_to_fp64_mid": round(h / mid - 1, 5),
+                       "inside": bool(lo <= h <= hi)}
+                if ("_f32d", it) in other:
+                    row["oracle_f32_fp64_scalars"] = other[("_f32d", it)][0]
+                if ("_f32", it) in other:
+                    row["oracle_f32_min"] = min(other[("_f32", it)])
+                    row["oracle_f32_max"] = max(other[("_f32", it)])
+                rows.append(row)
         out["trace_vs_fp64"] = rows
     return out
 
