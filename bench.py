@@ -65,10 +65,13 @@ def kernel_bytes(name, d, grid, cells_pins):
         "pass2_col_spectral": B * 16,
         # read three maps (12) + density (4), write field x/y and potential (12)
         "pass3_row_final": B * 28,
-        # per cell: CSR start (4), position/offset/size/ratio (28), pins/area (8),
-        # previous position and gradient (16), gradient write (8); per pin: slot id (4) +
-        # pin gradient (8); field maps read once (8 per bin)
-        "cell_fused": C * (4 + 28 + 8 + 16 + 8) + cells_pins * 12 + B * 8,
+        # per sorted cell: record (16) and ratio/id record (16) read, field force
+        # written to the cell's id (8); field maps read once (8 per bin)
+        "field_sorted": C * 40 + B * 8,
+        # per cell: CSR start (4), position (8), ratio/pins/area (16), previous position
+        # and gradient (16), field force (8), gradient write (8); per pin: slot id (4) +
+        # pin gradient (8)
+        "cell_fused": C * (4 + 8 + 16 + 16 + 8 + 8) + cells_pins * 12,
         # position, previous position, gradient, clamp bound read; three writes
         "update": C * 56,
     }

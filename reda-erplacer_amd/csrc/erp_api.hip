@@ -205,8 +205,13 @@ int launch_fused_iteration(erp_ctx *ctx, bool fork = false) {
         if ((rc = launch_wa(djob(ctx), ctx->dims, 1, ctx->stream, t))) return rc;
         if ((rc = run_potential(ctx))) return rc;
     }
+    // every form above ran this iteration's sort: the field gather follows its order
+    if (t) t->begin("field_sorted");
+    rc = launch_field_sorted(djob(ctx), ctx->dims, 1, ctx->stream);
+    if (t) t->end();
+    if (rc) return rc;
     if (t) t->begin("cell_fused");
-    rc = launch_cell_fused(djob(ctx), ctx->dims, 1, ctx->debug_parts, ctx->stream, fork);
+    rc = launch_cell_fused(djob(ctx), ctx->dims, 1, ctx->debug_parts, ctx->stream, fork, /*read_gd=*/true);
     if (t) t->end();
     if (rc) return rc;
     if (t) t->begin("scalar");
@@ -287,7 +292,8 @@ int erp::launch_batched_iteration(const JobDesc *d, const BatchDims &bd, int nj,
     int rc = launch_sort(d, bd, nj, owner, s);
     if (!rc) rc = owner ? launch_scatter_owner(d, bd, nj, s) : launch_scatter(d, bd, nj, true, s);
     if (!rc) rc = launch_potential_fft(d, bd, nj, s, nullptr, /*with_wa=*/true);   // WA inside pass 1
-    if (!rc) rc = launch_cell_fused(d, bd, nj, false, s);
+    if (!rc) rc = launch_field_sorted(d, bd, nj, s);
+    if (!rc) rc = launch_cell_fused(d, bd, nj, false, s, false, /*read_gd=*/true);
     if (!rc) rc = launch_scalar(d, nj, SC_HPWL | SC_POT | SC_NEST | SC_SCHED, s);
     if (!rc) rc = launch_update(d, bd, nj, s);
     return rc;

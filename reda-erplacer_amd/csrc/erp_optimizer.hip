@@ -1,6 +1,8 @@
 // Per-cell kernels (field gather, WA pin->cell gather, preconditioned Nesterov
 // step) and the device-side scalar state (Nesterov momentum/step size and the
 // Scheduler), gfx950.
+#include <climits>
+
 #include "erp_internal.h"
 #include "erp_kernels.h"
 
@@ -27,22 +29,28 @@ __device__ __forceinline__ float tri_unclamped(float x, float s, int k, float bs
     return fminf(x + s, bin_xl + bs) - fmaxf(x, bin_xl);
 }
 
-// eforce.rs:40-90 for one cell: -ratio * sum_b overlap_x overlap_y field_b,
-// bins [max(trunc(x/bw), 0), min(trunc((x+s)/bw) + 1, M)).
-__device__ __forceinline__ float2 efield_cell(const CellBufs &c, int64_t i, float inv_bw,
-                                              float inv_bh) {
-    const float2 p = gld(c.pos + i);
-    const float4 gq = gld(c.cgeo + i);
-    const float2 o = make_float2(gq.x, gq.y), sz = make_float2(gq.z, gq.w);
-    const float nx = p.x + o.x, ny = p.y + o.y;
-    int xl = (int)((nx - 0.f) * inv_bw);
+// The bins [xl, xh) x [yl, yh) of a cell's field integral (eforce.rs:40-90):
+// [max(trunc(x/bw), 0), min(trunc((x+s)/bw) + 1, M)) per axis, given the cell's
+// clamped lower-left corner (nx, ny) = position + clamp offset and clamped size.
+__device__ __forceinline__ void field_window(const CellBufs &c, float nx, float ny, float2 sz,
+                                             float inv_bw, float inv_bh, int &xl, int &xh,
+                                             int &yl, int &yh) {
+    xl = (int)((nx - 0.f) * inv_bw);
     xl = xl < 0 ? 0 : xl;
-    int xh = (int)((nx + sz.x - 0.f) * inv_bw) + 1;
+    xh = (int)((nx + sz.x - 0.f) * inv_bw) + 1;
     xh = xh > c.m ? c.m : xh;
-    int yl = (int)((ny - 0.f) * inv_bh);
+    yl = (int)((ny - 0.f) * inv_bh);
     yl = yl < 0 ? 0 : yl;
-    int yh = (int)((ny + sz.y - 0.f) * inv_bh) + 1;
+    yh = (int)((ny + sz.y - 0.f) * inv_bh) + 1;
     yh = yh > c.n ? c.n : yh;
+}
+
+// eforce.rs:40-90 for one cell: -ratio * sum_b overlap_x overlap_y field_b over
+// its window, summed row by row (k), each row in increasing h.
+__device__ __forceinline__ float2 efield_at(const CellBufs &c, float nx, float ny, float2 sz,
+                                            float r, float inv_bw, float inv_bh) {
+    int xl, xh, yl, yh;
+    field_window(c, nx, ny, sz, inv_bw, inv_bh, xl, xh, yl, yh);
     float gx = 0.f, gy = 0.f;
     if (yh - yl <= 5) {
         // the window's columns lie in the 4-bin block at or below yl and the
@@ -83,8 +91,16 @@ __device__ __forceinline__ float2 efield_cell(const CellBufs &c, int64_t i, floa
             }
         }
     }
-    const float r = gld(c.cprop + i).x;
     return make_float2(-gx * r, -gy * r);
+}
+
+// the field force of cell i in id order (its inputs from pos, cgeo, cprop)
+__device__ __forceinline__ float2 efield_cell(const CellBufs &c, int64_t i, float inv_bw,
+                                              float inv_bh) {
+    const float2 p = gld(c.pos + i);
+    const float4 gq = gld(c.cgeo + i);
+    const float r = gld(c.cprop + i).x;
+    return efield_at(c, p.x + gq.x, p.y + gq.y, make_float2(gq.z, gq.w), r, inv_bw, inv_bh);
 }
 
 // hpwl.rs:120-134: sum of the cell's pin gradients in macro_2_pins order
@@ -136,6 +152,97 @@ __global__ void __launch_bounds__(kBlock) k_efield(const JobDesc *__restrict__ j
     if (i < c.C) c.gd[i] = efield_cell(c, i, 1.f / c.bw, 1.f / c.bh);
 }
 
+// The field gather in the spatial order of this iteration's sort: sorted
+// record i holds the cell's clamped corner and size (rec) and its ratio and id
+// (aux), the values efield_cell derives from pos, cgeo and cprop (the same f32
+// sums).  A workgroup's windows then lie in one region of the map: it copies
+// the 4 x 4-bin blocks that its cells' windows touch into LDS once (whole
+// 128-byte lines, coalesced) and every cell sums its window from there, in the
+// order of efield_at.  Cells with windows above kFieldSmall bins a side, and
+// every cell of a workgroup whose box exceeds kFieldBoxBlocks, read the map
+// directly.  The force is written to the cell's id.
+constexpr int kFieldBoxBlocks = 128;   // LDS box: 4 x 4-bin blocks (16 KB)
+constexpr int kFieldSmall = 8;         // window side (bins) of the cells that read the box
+
+__device__ __forceinline__ void block_minmax(int &lo, int &hi, int (*red)[kBlock / 64], int slot) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, __shfl_xor(lo, off));
+        hi = max(hi, __shfl_xor(hi, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[slot][threadIdx.x >> 6] = lo;
+        red[slot + 1][threadIdx.x >> 6] = hi;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_field_sorted(const JobDesc *__restrict__ jd) {
+    __shared__ float4 box[kFieldBoxBlocks * 8];
+    __shared__ int red[4][kBlock / 64];
+    if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
+    const CellBufs &c = jd[blockIdx.y].cb;
+    const int64_t nblk = (c.C + kBlock - 1) / kBlock;
+    if ((int64_t)blockIdx.x >= nblk) return;                // uniform per workgroup
+    const int64_t i = xcd_block(blockIdx.x, nblk) * kBlock + threadIdx.x;
+    const bool valid = i < c.C;
+    const SortBufs &sb = jd[blockIdx.y].sb;
+    float4 r4 = make_float4(0.f, 0.f, 0.f, 0.f), a4 = r4;
+    if (valid) {
+        r4 = gld(sb.rec + i);
+        a4 = gld(sb.aux + i);
+    }
+    const float inv_bw = 1.f / c.bw, inv_bh = 1.f / c.bh;
+    const float2 sz = make_float2(r4.z, r4.w);
+    int xl = 0, xh = 0, yl = 0, yh = 0;
+    if (valid) field_window(c, r4.x, r4.y, sz, inv_bw, inv_bh, xl, xh, yl, yh);
+    const bool small = valid && xh > xl && yh > yl && xh - xl <= kFieldSmall && yh - yl <= kFieldSmall;
+    // the box of the small windows, in 4 x 4-bin blocks [lx, hx] x [ly, hy]
+    int lx = small ? xl >> 2 : INT_MAX, hx = small ? (xh - 1) >> 2 : INT_MIN;
+    int ly = small ? yl >> 2 : INT_MAX, hy = small ? (yh - 1) >> 2 : INT_MIN;
+    block_minmax(lx, hx, red, 0);
+    block_minmax(ly, hy, red, 2);
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        lx = min(lx, red[0][w]);
+        hx = max(hx, red[1][w]);
+        ly = min(ly, red[2][w]);
+        hy = max(hy, red[3][w]);
+    }
+    const int wxb = hx >= lx ? hx - lx + 1 : 0, wyb = hy >= ly ? hy - ly + 1 : 0;
+    const bool use_box = wxb > 0 && wyb > 0 && wxb * wyb <= kFieldBoxBlocks;   // uniform
+    if (use_box) {
+        const int nbr = (c.n + 3) >> 2;                    // blocks per block row of the map
+        const float4 *src = reinterpret_cast<const float4 *>(c.fxy);
+        for (int t = threadIdx.x; t < wxb * wyb * 8; t += kBlock) {
+            const int b = t >> 3;
+            const int kb = lx + b / wyb, hb = ly + b % wyb;
+            box[t] = gld(src + ((size_t)kb * nbr + hb) * 8 + (t & 7));
+        }
+        __syncthreads();
+    }
+    if (!valid) return;
+    float2 gd;
+    if (use_box && small) {
+        const float2 *bins = reinterpret_cast<const float2 *>(box);
+        float gx = 0.f, gy = 0.f;
+        for (int k = xl; k < xh; ++k) {
+            const float px = tri_unclamped(r4.x, sz.x, k, c.bw);
+            const int row = ((k >> 2) - lx) * wyb;
+            for (int h = yl; h < yh; ++h) {
+                const float2 v = bins[(row + (h >> 2) - ly) * 16 + (((k & 3) << 2) | (h & 3))];
+                const float area = px * tri_unclamped(r4.y, sz.y, h, c.bh);
+                gx += area * v.x;
+                gy += area * v.y;
+            }
+        }
+        gd = make_float2(-gx * a4.z, -gy * a4.z);
+    } else {
+        gd = efield_at(c, r4.x, r4.y, sz, a4.z, inv_bw, inv_bh);
+    }
+    c.gd[__float_as_int(a4.w)] = gd;
+}
+
 __global__ void __launch_bounds__(kBlock) k_l1(const JobDesc *__restrict__ jd) {
     __shared__ double red[kBlock / 64];
     ERP_CELL_JOB();
@@ -183,7 +290,9 @@ __global__ void __launch_bounds__(kBlock) k_combine(const JobDesc *__restrict__ 
 
 // READ_GWL: the WA gradient per cell was summed already (k_pin2cell, on the
 // graph branch that runs beside the spectral passes) and is read from gwl.
-template <bool STORE_PARTS, bool READ_GWL>
+// READ_GD: the field force was gathered already (k_field_sorted) and is read
+// from gd.
+template <bool STORE_PARTS, bool READ_GWL, bool READ_GD>
 __global__ void __launch_bounds__(kBlock) k_cell_fused(const JobDesc *__restrict__ jd) {
     __shared__ double red[kBlock / 64];
     ERP_CELL_JOB();
@@ -191,12 +300,12 @@ __global__ void __launch_bounds__(kBlock) k_cell_fused(const JobDesc *__restrict
     double a = 0.0, b = 0.0;
     if (i < c.C) {
         const float2 gwl = READ_GWL ? gld(c.gwl + i) : pin2cell(c, i);
-        const float2 gd = efield_cell(c, i, 1.f / c.bw, 1.f / c.bh);
+        const float2 gd = READ_GD ? gld(c.gd + i) : efield_cell(c, i, 1.f / c.bw, 1.f / c.bh);
         const float2 g = combine(c, i, gwl, gd, (float)st->density_weight);
         c.g[i] = g;
         if (STORE_PARTS) {
             if (!READ_GWL) c.gwl[i] = gwl;
-            c.gd[i] = gd;
+            if (!READ_GD) c.gd[i] = gd;
         }
         bb_terms(c, i, g, a, b);
     }
@@ -464,14 +573,25 @@ int launch_l1(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s) { ER
 int launch_combine(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s) {
     ERP_LAUNCH_CELLS(k_combine, d);
 }
+int launch_field_sorted(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s) {
+    ERP_LAUNCH_CELLS(k_field_sorted, d);
+}
 int launch_cell_fused(const JobDesc *d, const BatchDims &bd, int nj, bool store_parts,
-                      hipStream_t s, bool read_gwl) {
-    if (read_gwl) {
-        if (store_parts) ERP_LAUNCH_CELLS((k_cell_fused<true, true>), d);
-        ERP_LAUNCH_CELLS((k_cell_fused<false, true>), d);
+                      hipStream_t s, bool read_gwl, bool read_gd) {
+    if (read_gd) {
+        if (read_gwl) {
+            if (store_parts) ERP_LAUNCH_CELLS((k_cell_fused<true, true, true>), d);
+            ERP_LAUNCH_CELLS((k_cell_fused<false, true, true>), d);
+        }
+        if (store_parts) ERP_LAUNCH_CELLS((k_cell_fused<true, false, true>), d);
+        ERP_LAUNCH_CELLS((k_cell_fused<false, false, true>), d);
     }
-    if (store_parts) ERP_LAUNCH_CELLS((k_cell_fused<true, false>), d);
-    ERP_LAUNCH_CELLS((k_cell_fused<false, false>), d);
+    if (read_gwl) {
+        if (store_parts) ERP_LAUNCH_CELLS((k_cell_fused<true, true, false>), d);
+        ERP_LAUNCH_CELLS((k_cell_fused<false, true, false>), d);
+    }
+    if (store_parts) ERP_LAUNCH_CELLS((k_cell_fused<true, false, false>), d);
+    ERP_LAUNCH_CELLS((k_cell_fused<false, false, false>), d);
 }
 int launch_update(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s) {
     ERP_LAUNCH_CELLS(k_update, d);
