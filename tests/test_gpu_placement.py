@@ -129,16 +129,21 @@ def test_graph_loop_matches_phase_loop(gpu_lib, small):
     assert A.info()["hpwl"] == B.info()["hpwl"]
 
 
-def test_graph_loop_matches_phase_loop_capped_wa(gpu_lib):
-    """At 350k cells the graph's side-branch WA launch (capped at 512 workgroups)
-    loops over a plan of more than twice as many workgroups; the loop is still
-    bit-identical to the phase kernels (uncapped WA, one workgroup per block)."""
+@pytest.mark.parametrize("mode", ["fft", "mfma"])
+def test_graph_loop_matches_phase_loop_capped_wa(gpu_lib, mode):
+    """At 350k cells the WA plan has more than twice as many workgroups as the
+    MFMA form's side-branch WA launch (capped at 512 workgroups, looping over the
+    plan); the FFT form runs the whole plan inside pass 1's launch.  Either loop
+    is bit-identical to the phase kernels (one WA launch, one workgroup per
+    block; the field gather in id order)."""
+    from erplacer import lib as L
     from erplacer import workloads as W
     from erplacer.placement import Placement
     d, g, _, td = W.make("large", scale=0.35)
     assert math.ceil(d.n_2nets / 256) > 512        # the two-pin blocks alone exceed the cap
-    A = Placement(d, g, target_density=td)
-    B = Placement(d, g, target_density=td)
+    sm = L.SPECTRAL_FFT if mode == "fft" else L.SPECTRAL_MFMA_BF16X3
+    A = Placement(d, g, target_density=td, spectral_mode=sm)
+    B = Placement(d, g, target_density=td, spectral_mode=sm)
     for _ in range(17):
         A.step()
     B.run(17)
@@ -148,6 +153,52 @@ def test_graph_loop_matches_phase_loop_capped_wa(gpu_lib):
     assert A.info()["hpwl"] == B.info()["hpwl"]
     A.close()
     B.close()
+
+
+@pytest.mark.parametrize("cells,grid,spread", [(20000, 128, False), (20000, 128, True),
+                                               (4000, 512, True)])
+def test_sorted_field_gather_matches_id_order(gpu_lib, cells, grid, spread):
+    """The loop gathers the field in the sort's spatial order (k_field_sorted:
+    each workgroup's 4 x 4-bin blocks staged in LDS; cells with windows above 8
+    bins a side, and workgroups whose box exceeds 128 blocks, read the map
+    directly), the phase path in id order (k_efield).  After the same steps the
+    positions and the stored field forces are bit-identical.  20k cells at 128^2
+    (1.2 per bin, centre start and spread): boxes of one or two sort tiles in
+    LDS; 4k cells at 512^2 spread: ~4 cells per 16 x 16-bin sort tile, so a
+    workgroup's 256 cells span far more than 128 blocks, and every cell is more
+    than 8 bins tall (the direct reads, over the general window loop)."""
+    from erplacer import lib as L
+    from erplacer.netlist import generate
+    from erplacer.placement import Placement
+    d = generate(cells, int(cells * 1.05), 32, 8, 0.10, 0.7, 11)
+    C = d.num_movable
+    with Placement(d, grid, target_density=1.0) as A, Placement(d, grid, target_density=1.0) as B:
+        st = A.setup()
+        span = np.maximum(st["cw"][:C] / st["bw"], st["ch"][:C] / st["bh"])
+        tiles = (grid // 16) ** 2
+        if grid == 512:
+            assert C / tiles < 8 and (span > 8).all()          # the direct-read cases occur
+        else:
+            assert C / tiles > 256 and span.max() < 4           # every window fits a box
+        if spread:
+            rng = np.random.default_rng(3)
+            x = d.x.astype(np.float32).copy()
+            y = d.y.astype(np.float32).copy()
+            x[:C] = rng.uniform(0, d.die_w - d.w[:C] - 1).astype(np.float32)
+            y[:C] = rng.uniform(0, d.die_h - d.h[:C] - 1).astype(np.float32)
+            A.set_positions(x, y)
+            B.set_positions(x, y)
+        A.set_store_gradients(True)
+        B.set_store_gradients(True)
+        for _ in range(12):
+            A.step()
+        B.run(12)
+        xa, ya = A.positions()
+        xb, yb = B.positions()
+        assert np.array_equal(xa, xb) and np.array_equal(ya, yb)
+        ga, gb = A.cell_grads(L.GRAD_DENSITY), B.cell_grads(L.GRAD_DENSITY)
+        assert np.array_equal(ga[0], gb[0]) and np.array_equal(ga[1], gb[1])
+        assert np.abs(ga[0]).max() > 0
 
 
 def test_deterministic(gpu_lib, small):
