@@ -205,9 +205,11 @@ int launch_fused_iteration(erp_ctx *ctx, bool fork = false) {
         if ((rc = launch_wa(djob(ctx), ctx->dims, 1, ctx->stream, t))) return rc;
         if ((rc = run_potential(ctx))) return rc;
     }
-    // every form above ran this iteration's sort: the field gather follows its order
+    // every form above ran this iteration's sort: the field gather follows its
+    // order; the step's HPWL / overflow / energy and the scheduler's staged half
+    // run beside it, the Nesterov update and the commit after the cell kernel
     if (t) t->begin("field_sorted");
-    rc = launch_field_sorted(djob(ctx), ctx->dims, 1, ctx->stream);
+    rc = launch_field_sorted(djob(ctx), ctx->dims, 1, ctx->stream, /*with_sched=*/true);
     if (t) t->end();
     if (rc) return rc;
     if (t) t->begin("cell_fused");
@@ -215,7 +217,7 @@ int launch_fused_iteration(erp_ctx *ctx, bool fork = false) {
     if (t) t->end();
     if (rc) return rc;
     if (t) t->begin("scalar");
-    rc = launch_scalar(djob(ctx), 1, SC_HPWL | SC_POT | SC_NEST | SC_SCHED, ctx->stream);
+    rc = launch_scalar(djob(ctx), 1, SC_NEST | SC_COMMIT, ctx->stream);
     if (t) t->end();
     if (rc) return rc;
     if (t) t->begin("update");
@@ -292,9 +294,9 @@ int erp::launch_batched_iteration(const JobDesc *d, const BatchDims &bd, int nj,
     int rc = launch_sort(d, bd, nj, owner, s);
     if (!rc) rc = owner ? launch_scatter_owner(d, bd, nj, s) : launch_scatter(d, bd, nj, true, s);
     if (!rc) rc = launch_potential_fft(d, bd, nj, s, nullptr, /*with_wa=*/true);   // WA inside pass 1
-    if (!rc) rc = launch_field_sorted(d, bd, nj, s);
+    if (!rc) rc = launch_field_sorted(d, bd, nj, s, /*with_sched=*/true);
     if (!rc) rc = launch_cell_fused(d, bd, nj, false, s, false, /*read_gd=*/true);
-    if (!rc) rc = launch_scalar(d, nj, SC_HPWL | SC_POT | SC_NEST | SC_SCHED, s);
+    if (!rc) rc = launch_scalar(d, nj, SC_NEST | SC_COMMIT, s);
     if (!rc) rc = launch_update(d, bd, nj, s);
     return rc;
 }

@@ -121,6 +121,9 @@ struct DevState {
     // reductions of the current step
     double hpwl, overflow, energy, l1_wl, l1_d, num, den;
     int32_t nonfinite, should_stop;
+    // the scheduler's next density weight and gamma, computed ahead of the
+    // Nesterov update of the same step and applied after it
+    double dw_next, gamma_next;
 };
 
 // Spectral transform tables for one length L (power of two).

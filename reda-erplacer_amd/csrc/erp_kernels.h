@@ -147,7 +147,15 @@ enum ScalarOps {
     SC_POT = 2,     // overflow, energy from their partials
     SC_L1 = 4,      // density_weight *= l1_wl / l1_d (iteration 0)
     SC_NEST = 8,    // update_momentum, then BB step size
-    SC_SCHED = 16,  // Scheduler::update
+    SC_SCHED = 16,  // Scheduler::update (SC_SCHED_STAGE, then SC_COMMIT)
+    // Scheduler::update in two halves around the Nesterov update of the same
+    // step.  STAGE: the history, should_stop, and the next density weight and
+    // gamma into dw_next / gamma_next (it reads the density weight, the step's
+    // HPWL and overflow and the iteration count, none of which SC_NEST writes,
+    // and writes nothing SC_NEST reads).  COMMIT: iteration + 1 and the staged
+    // density weight and gamma.
+    SC_SCHED_STAGE = 32,
+    SC_COMMIT = 64,
 };
 struct ScalarArgs {
     DevState *st;
@@ -216,8 +224,11 @@ int launch_efield(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
 int launch_l1(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
 int launch_combine(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
 // Field force per cell gathered in the sorted order of this iteration's sort
-// (rec / aux must hold it), written to gd in id order.
-int launch_field_sorted(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
+// (rec / aux must hold it), written to gd in id order.  with_sched: one more
+// workgroup per job runs launch_scalar's SC_HPWL | SC_POT | SC_SCHED_STAGE
+// beside it (the WA, overflow and energy partials must be complete).
+int launch_field_sorted(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
+                        bool with_sched = false);
 // read_gwl: the per-cell WA gradient was summed by k_pin2cell already;
 // read_gd: the field force was gathered by launch_field_sorted already
 int launch_cell_fused(const JobDesc *d, const BatchDims &bd, int nj, bool store_parts,

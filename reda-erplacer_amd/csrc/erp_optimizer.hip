@@ -176,73 +176,6 @@ __device__ __forceinline__ void block_minmax(int &lo, int &hi, int (*red)[kBlock
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_field_sorted(const JobDesc *__restrict__ jd) {
-    __shared__ float4 box[kFieldBoxBlocks * 8];
-    __shared__ int red[4][kBlock / 64];
-    if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
-    const CellBufs &c = jd[blockIdx.y].cb;
-    const int64_t nblk = (c.C + kBlock - 1) / kBlock;
-    if ((int64_t)blockIdx.x >= nblk) return;                // uniform per workgroup
-    const int64_t i = xcd_block(blockIdx.x, nblk) * kBlock + threadIdx.x;
-    const bool valid = i < c.C;
-    const SortBufs &sb = jd[blockIdx.y].sb;
-    float4 r4 = make_float4(0.f, 0.f, 0.f, 0.f), a4 = r4;
-    if (valid) {
-        r4 = gld(sb.rec + i);
-        a4 = gld(sb.aux + i);
-    }
-    const float inv_bw = 1.f / c.bw, inv_bh = 1.f / c.bh;
-    const float2 sz = make_float2(r4.z, r4.w);
-    int xl = 0, xh = 0, yl = 0, yh = 0;
-    if (valid) field_window(c, r4.x, r4.y, sz, inv_bw, inv_bh, xl, xh, yl, yh);
-    const bool small = valid && xh > xl && yh > yl && xh - xl <= kFieldSmall && yh - yl <= kFieldSmall;
-    // the box of the small windows, in 4 x 4-bin blocks [lx, hx] x [ly, hy]
-    int lx = small ? xl >> 2 : INT_MAX, hx = small ? (xh - 1) >> 2 : INT_MIN;
-    int ly = small ? yl >> 2 : INT_MAX, hy = small ? (yh - 1) >> 2 : INT_MIN;
-    block_minmax(lx, hx, red, 0);
-    block_minmax(ly, hy, red, 2);
-    __syncthreads();
-#pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) {
-        lx = min(lx, red[0][w]);
-        hx = max(hx, red[1][w]);
-        ly = min(ly, red[2][w]);
-        hy = max(hy, red[3][w]);
-    }
-    const int wxb = hx >= lx ? hx - lx + 1 : 0, wyb = hy >= ly ? hy - ly + 1 : 0;
-    const bool use_box = wxb > 0 && wyb > 0 && wxb * wyb <= kFieldBoxBlocks;   // uniform
-    if (use_box) {
-        const int nbr = (c.n + 3) >> 2;                    // blocks per block row of the map
-        const float4 *src = reinterpret_cast<const float4 *>(c.fxy);
-        for (int t = threadIdx.x; t < wxb * wyb * 8; t += kBlock) {
-            const int b = t >> 3;
-            const int kb = lx + b / wyb, hb = ly + b % wyb;
-            box[t] = gld(src + ((size_t)kb * nbr + hb) * 8 + (t & 7));
-        }
-        __syncthreads();
-    }
-    if (!valid) return;
-    float2 gd;
-    if (use_box && small) {
-        const float2 *bins = reinterpret_cast<const float2 *>(box);
-        float gx = 0.f, gy = 0.f;
-        for (int k = xl; k < xh; ++k) {
-            const float px = tri_unclamped(r4.x, sz.x, k, c.bw);
-            const int row = ((k >> 2) - lx) * wyb;
-            for (int h = yl; h < yh; ++h) {
-                const float2 v = bins[(row + (h >> 2) - ly) * 16 + (((k & 3) << 2) | (h & 3))];
-                const float area = px * tri_unclamped(r4.y, sz.y, h, c.bh);
-                gx += area * v.x;
-                gy += area * v.y;
-            }
-        }
-        gd = make_float2(-gx * a4.z, -gy * a4.z);
-    } else {
-        gd = efield_at(c, r4.x, r4.y, sz, a4.z, inv_bw, inv_bh);
-    }
-    c.gd[__float_as_int(a4.w)] = gd;
-}
-
 __global__ void __launch_bounds__(kBlock) k_l1(const JobDesc *__restrict__ jd) {
     __shared__ double red[kBlock / 64];
     ERP_CELL_JOB();
@@ -336,10 +269,15 @@ __global__ void __launch_bounds__(kBlock) k_update(const JobDesc *__restrict__ j
     c.prev_g[i] = g;
 }
 
-// Deterministic sums of the partial arrays by one workgroup, then the FP64
-// Nesterov and scheduler updates (one workgroup per job).
+// Deterministic sums of the partial arrays, then the FP64 Nesterov and
+// scheduler updates, by one workgroup per job.  The sums are those of
+// kScalarThreads virtual threads: virtual thread v adds the entries
+// k = v (mod kScalarThreads) in increasing k, the lanes of each virtual
+// wavefront are summed by butterfly, then the wavefronts in order.  A workgroup
+// of T threads carries kScalarThreads / T virtual threads per thread (thread t
+// the virtual threads t + T j, which sit in lane t mod 64 of virtual wavefront
+// t / 64 + (T / 64) j), so the sums have the same bits for every T.
 constexpr int kScalarThreads = 1024;
-constexpr int kScalarUnroll = 8;
 
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
@@ -352,10 +290,15 @@ __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
     return v;
 }
 
-__global__ void __launch_bounds__(kScalarThreads) k_scalar(const JobDesc *__restrict__ jd, int ops) {
-    if (gridDim.x > 1 && !jd[blockIdx.x].active) return;   // an empty slot of a batch
-    const ScalarArgs &a = jd[blockIdx.x].sa;
-    constexpr int kWaves = kScalarThreads / 64;
+// LOADS: loads per array in flight per thread (the order of the sums does not
+// depend on it).
+template <int T, int LOADS>
+__device__ __forceinline__ void scalar_body(const ScalarArgs &a, int ops) {
+    constexpr int V = kScalarThreads / T;            // virtual threads per thread
+    constexpr int U = LOADS / V;                     // strides per round
+    static_assert(V * T == kScalarThreads && U * V == LOADS && T % 64 == 0 && T >= 256,
+                  "scalar workgroup shape");
+    constexpr int kWaves = kScalarThreads / 64;      // virtual wavefronts
     __shared__ double redd[4][kWaves];
     __shared__ int64_t redi[kWaves];
     __shared__ double tot[4];
@@ -364,16 +307,17 @@ __global__ void __launch_bounds__(kScalarThreads) k_scalar(const JobDesc *__rest
     // the device state is staged in LDS: its words are loaded together with the
     // partials, updated there by one thread and stored back together
     constexpr int kStateWords = (int)(sizeof(DevState) / 8);
-    static_assert(sizeof(DevState) % 8 == 0, "DevState is copied as 8-byte words");
+    static_assert(sizeof(DevState) % 8 == 0 && kStateWords <= T, "DevState is copied as 8-byte words");
     __shared__ unsigned long long ssw[kStateWords];
     const unsigned long long *gsw = reinterpret_cast<const unsigned long long *>(a.st);
     const unsigned long long sword = threadIdx.x < kStateWords ? gld(gsw + threadIdx.x) : 0ull;
-    // every thread sums a fixed strided subset of each partial array, in
-    // increasing index; the loads of kScalarUnroll strides of all arrays are
-    // issued before any is added (one memory round trip per kScalarUnroll
-    // strides, not one per stride)
-    double h = 0, e = 0, pa = 0, pb = 0;
-    int64_t o = 0;
+    double h[V], e[V], pa[V], pb[V];
+    int64_t o[V];
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+        h[j] = e[j] = pa[j] = pb[j] = 0.0;
+        o[j] = 0;
+    }
     const int nh = (ops & SC_HPWL) ? a.n_hpwl : 0;
     const int no = (ops & SC_POT) ? a.n_ovf : 0;
     const int ne = (ops & SC_POT) ? a.n_energy : 0;
@@ -381,31 +325,43 @@ __global__ void __launch_bounds__(kScalarThreads) k_scalar(const JobDesc *__rest
     const int most = max(max(nh, no), max(ne, nc));
     const double *hp = a.hpwl_part, *ep = a.energy_part, *ap = a.part_a, *bp = a.part_b;
     const int64_t *op = a.ovf_part;
-    for (int k0 = threadIdx.x; k0 < most; k0 += kScalarUnroll * kScalarThreads) {
-        double vh[kScalarUnroll], ve[kScalarUnroll], va[kScalarUnroll], vb[kScalarUnroll];
-        int64_t vo[kScalarUnroll];
+    // U strides of every array (LOADS loads per thread) are issued before any
+    // is added
+    for (int k0 = 0; k0 < most; k0 += U * kScalarThreads) {
+        double vh[U][V], ve[U][V], va[U][V], vb[U][V];
+        int64_t vo[U][V];
 #pragma unroll
-        for (int u = 0; u < kScalarUnroll; ++u) {
-            const int k = k0 + u * kScalarThreads;
-            vh[u] = k < nh ? gld(hp + k) : 0.0;
-            vo[u] = k < no ? gld(op + k) : 0;
-            ve[u] = k < ne ? gld(ep + k) : 0.0;
-            va[u] = k < nc ? gld(ap + k) : 0.0;
-            vb[u] = k < nc ? gld(bp + k) : 0.0;
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const int k = k0 + u * kScalarThreads + (int)threadIdx.x + T * j;
+                vh[u][j] = k < nh ? gld(hp + k) : 0.0;
+                vo[u][j] = k < no ? gld(op + k) : 0;
+                ve[u][j] = k < ne ? gld(ep + k) : 0.0;
+                va[u][j] = k < nc ? gld(ap + k) : 0.0;
+                vb[u][j] = k < nc ? gld(bp + k) : 0.0;
+            }
         }
 #pragma unroll
-        for (int u = 0; u < kScalarUnroll; ++u) {
-            h += vh[u]; o += vo[u]; e += ve[u]; pa += va[u]; pb += vb[u];
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                h[j] += vh[u][j]; o[j] += vo[u][j]; e[j] += ve[u][j]; pa[j] += va[u][j]; pb[j] += vb[u][j];
+            }
         }
     }
     if (threadIdx.x < kStateWords) ssw[threadIdx.x] = sword;
-    // fixed-order reduction: lanes of a wavefront by butterfly, then the waves in order
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    h = wave_sum_f64(h); e = wave_sum_f64(e); pa = wave_sum_f64(pa); pb = wave_sum_f64(pb);
-    o = wave_sum_i64(o);
-    if (lane == 0) {
-        redd[0][wave] = h; redd[1][wave] = e; redd[2][wave] = pa; redd[3][wave] = pb;
-        redi[wave] = o;
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+        h[j] = wave_sum_f64(h[j]); e[j] = wave_sum_f64(e[j]);
+        pa[j] = wave_sum_f64(pa[j]); pb[j] = wave_sum_f64(pb[j]);
+        o[j] = wave_sum_i64(o[j]);
+        if (lane == 0) {
+            const int w = wave + (T / 64) * j;
+            redd[0][w] = h[j]; redd[1][w] = e[j]; redd[2][w] = pa[j]; redd[3][w] = pb[j];
+            redi[w] = o[j];
+        }
     }
     __syncthreads();
     if (threadIdx.x < 4) {
@@ -421,9 +377,10 @@ __global__ void __launch_bounds__(kScalarThreads) k_scalar(const JobDesc *__rest
     const double hsum = tot[0], esum = tot[1], asum = tot[2], bsum = tot[3];
     const int64_t osum = toti;
     DevState *sp = reinterpret_cast<DevState *>(ssw);
+    const bool sched = (ops & (SC_SCHED | SC_SCHED_STAGE)) != 0;
     // scheduler.rs:92-130 needs three powers; lane 0 of waves 1..3 evaluates them
     // concurrently from the values the update below will see.
-    if ((ops & SC_SCHED) && lane == 0 && wave >= 1 && wave <= 3) {
+    if (sched && lane == 0 && wave >= 1 && wave <= 3) {
         const DevState &s = *sp;
         const double hcur = (ops & SC_HPWL) ? hsum : s.hpwl;
         const double ocur = (ops & SC_POT) ? (double)osum * a.inv_scale_over_area : s.overflow;
@@ -460,8 +417,7 @@ __global__ void __launch_bounds__(kScalarThreads) k_scalar(const JobDesc *__rest
             if (s.iteration > 0 && bsum > 0.0) s.alpha = sqrt(asum / bsum);
             s.dw_used = s.density_weight;
         }
-        if (ops & SC_SCHED) {                    // scheduler.rs:68-130
-            s.iteration += 1;
+        if (sched) {                             // scheduler.rs:68-130 but the iteration count
             if (!s.hpwl_has_initial) { s.hpwl_has_initial = 1; s.hpwl_initial = s.hpwl; }
             s.hpwl_prev = s.hpwl_cur;
             s.hpwl_cur = s.hpwl;
@@ -477,17 +433,108 @@ __global__ void __launch_bounds__(kScalarThreads) k_scalar(const JobDesc *__rest
                 power = pw[1];
                 power = power < 0.95 ? 0.95 : (power > 1.05 ? 1.05 : power);
             }
-            s.density_weight = s.density_weight * (1.05 * power);
+            s.dw_next = s.density_weight * (1.05 * power);
             double coef = pw[2];
             coef = coef < 0.05 ? 0.05 : (coef > 10.0 ? 10.0 : coef);
-            s.gamma = s.base_gamma * coef;
+            s.gamma_next = s.base_gamma * coef;
             s.should_stop = s.hpwl_prev != 0.0 && s.hpwl_cur != 0.0 && s.hpwl_prev > s.hpwl_cur * 1.5;
+        }
+        if (ops & (SC_SCHED | SC_COMMIT)) {      // scheduler.rs:69, and the values staged above
+            s.iteration += 1;
+            s.density_weight = s.dw_next;
+            s.gamma = s.gamma_next;
         }
         const double chk = s.hpwl + s.overflow + s.energy + s.alpha + s.beta + s.density_weight + s.gamma;
         s.nonfinite = isfinite(chk) ? 0 : 1;
     }
     __syncthreads();
     if (threadIdx.x < kStateWords) reinterpret_cast<unsigned long long *>(a.st)[threadIdx.x] = ssw[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kScalarThreads) k_scalar(const JobDesc *__restrict__ jd, int ops) {
+    if (gridDim.x > 1 && !jd[blockIdx.x].active) return;   // an empty slot of a batch
+    scalar_body<kScalarThreads, 8>(jd[blockIdx.x].sa, ops);
+}
+
+// WITH_SCHED: workgroup 0 runs the step's HPWL, overflow and energy sums and
+// the staged half of the scheduler (SC_HPWL | SC_POT | SC_SCHED_STAGE) beside
+// the gather, workgroups 1-7 are empty, and the gather's workgroups start at 8
+// (so that launch index mod 8 still names the XCD).  Every partial those sums
+// read is complete before this launch (WA and pass 1 to 3).
+template <bool WITH_SCHED>
+__global__ void __launch_bounds__(kBlock) k_field_sorted(const JobDesc *__restrict__ jd) {
+    __shared__ float4 box[kFieldBoxBlocks * 8];
+    __shared__ int red[4][kBlock / 64];
+    if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
+    int64_t bx = blockIdx.x;
+    if (WITH_SCHED) {
+        if (bx < 8) {                                      // uniform per workgroup
+            if (bx == 0) scalar_body<kBlock, 4>(jd[blockIdx.y].sa, SC_HPWL | SC_POT | SC_SCHED_STAGE);
+            return;
+        }
+        bx -= 8;
+    }
+    const CellBufs &c = jd[blockIdx.y].cb;
+    const int64_t nblk = (c.C + kBlock - 1) / kBlock;
+    if (bx >= nblk) return;                                 // uniform per workgroup
+    const int64_t i = xcd_block(bx, nblk) * kBlock + threadIdx.x;
+    const bool valid = i < c.C;
+    const SortBufs &sb = jd[blockIdx.y].sb;
+    float4 r4 = make_float4(0.f, 0.f, 0.f, 0.f), a4 = r4;
+    if (valid) {
+        r4 = gld(sb.rec + i);
+        a4 = gld(sb.aux + i);
+    }
+    const float inv_bw = 1.f / c.bw, inv_bh = 1.f / c.bh;
+    const float2 sz = make_float2(r4.z, r4.w);
+    int xl = 0, xh = 0, yl = 0, yh = 0;
+    if (valid) field_window(c, r4.x, r4.y, sz, inv_bw, inv_bh, xl, xh, yl, yh);
+    const bool small = valid && xh > xl && yh > yl && xh - xl <= kFieldSmall && yh - yl <= kFieldSmall;
+    // the box of the small windows, in 4 x 4-bin blocks [lx, hx] x [ly, hy]
+    int lx = small ? xl >> 2 : INT_MAX, hx = small ? (xh - 1) >> 2 : INT_MIN;
+    int ly = small ? yl >> 2 : INT_MAX, hy = small ? (yh - 1) >> 2 : INT_MIN;
+    block_minmax(lx, hx, red, 0);
+    block_minmax(ly, hy, red, 2);
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        lx = min(lx, red[0][w]);
+        hx = max(hx, red[1][w]);
+        ly = min(ly, red[2][w]);
+        hy = max(hy, red[3][w]);
+    }
+    const int wxb = hx >= lx ? hx - lx + 1 : 0, wyb = hy >= ly ? hy - ly + 1 : 0;
+    const bool use_box = wxb > 0 && wyb > 0 && wxb * wyb <= kFieldBoxBlocks;   // uniform
+    if (use_box) {
+        const int nbr = (c.n + 3) >> 2;                    // blocks per block row of the map
+        const float4 *src = reinterpret_cast<const float4 *>(c.fxy);
+        for (int t = threadIdx.x; t < wxb * wyb * 8; t += kBlock) {
+            const int b = t >> 3;
+            const int kb = lx + b / wyb, hb = ly + b % wyb;
+            box[t] = gld(src + ((size_t)kb * nbr + hb) * 8 + (t & 7));
+        }
+        __syncthreads();
+    }
+    if (!valid) return;
+    float2 gd;
+    if (use_box && small) {
+        const float2 *bins = reinterpret_cast<const float2 *>(box);
+        float gx = 0.f, gy = 0.f;
+        for (int k = xl; k < xh; ++k) {
+            const float px = tri_unclamped(r4.x, sz.x, k, c.bw);
+            const int row = ((k >> 2) - lx) * wyb;
+            for (int h = yl; h < yh; ++h) {
+                const float2 v = bins[(row + (h >> 2) - ly) * 16 + (((k & 3) << 2) | (h & 3))];
+                const float area = px * tri_unclamped(r4.y, sz.y, h, c.bh);
+                gx += area * v.x;
+                gy += area * v.y;
+            }
+        }
+        gd = make_float2(-gx * a4.z, -gy * a4.z);
+    } else {
+        gd = efield_at(c, r4.x, r4.y, sz, a4.z, inv_bw, inv_bh);
+    }
+    c.gd[__float_as_int(a4.w)] = gd;
 }
 
 // (hpwl, overflow, nonfinite) of every job of a batch into one array.
@@ -573,8 +620,12 @@ int launch_l1(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s) { ER
 int launch_combine(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s) {
     ERP_LAUNCH_CELLS(k_combine, d);
 }
-int launch_field_sorted(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s) {
-    ERP_LAUNCH_CELLS(k_field_sorted, d);
+int launch_field_sorted(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s, bool with_sched) {
+    if (!with_sched) ERP_LAUNCH_CELLS(k_field_sorted<false>, d);
+    const unsigned nb = bd.max_cells > 0 ? grid_for(bd.max_cells) : 0u;
+    hipLaunchKernelGGL(k_field_sorted<true>, dim3(8u + nb, nj), dim3(kBlock), 0, s, d);
+    ERP_HIP_TRY(hipGetLastError());
+    return ERP_OK;
 }
 int launch_cell_fused(const JobDesc *d, const BatchDims &bd, int nj, bool store_parts,
                       hipStream_t s, bool read_gwl, bool read_gd) {
