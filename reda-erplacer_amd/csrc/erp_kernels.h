@@ -34,6 +34,28 @@ __device__ __forceinline__ int64_t xcd_block(int64_t w, int64_t nb) {
     return x * q + (x < r ? x : r) + (w >> 3);
 }
 
+// Sum of v over a kBlock-thread workgroup in the order of the LDS tree
+// red[t] += red[t + s] for s = kBlock / 2, ..., 1, with the same bits: the two
+// levels across wavefronts through LDS, the six inside one wavefront by
+// shuffles (two barriers instead of nine).  The result is valid in thread 0.
+// red: kBlock elements of LDS, free again when the call returns.
+template <class T>
+__device__ __forceinline__ T block_sum_tree(T v, T *red) {
+    static_assert(kBlock == 256, "the tree's first two levels cross the four wavefronts");
+    red[threadIdx.x] = v;
+    __syncthreads();
+    T x = 0;
+    if (threadIdx.x < 64) {
+        const T a = red[threadIdx.x] + red[threadIdx.x + 128];        // level s = 128, t < 64
+        const T b = red[threadIdx.x + 64] + red[threadIdx.x + 192];   // level s = 128, t in [64, 128)
+        x = a + b;                                                     // level s = 64
+#pragma unroll
+        for (int s = 32; s > 0; s >>= 1) x += __shfl_down(x, s);     // levels 32 .. 1
+    }
+    __syncthreads();
+    return x;
+}
+
 // ------------------------------------------------------------ density
 // The sort and cell-major scatter kernels: 1024 cells per workgroup as 512
 // threads x 2 cells (at 1M cells 4141-4148 it/s as 256 x 4, 4272-4279 as

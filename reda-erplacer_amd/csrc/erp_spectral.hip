@@ -196,13 +196,8 @@ __device__ __forceinline__ void row_dct_body(
     }
     if (SRC_FIXED) {
         __shared__ int64_t red[kBlock];
-        red[threadIdx.x] = ovf;
-        __syncthreads();
-        for (int s = kBlock / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) ovf_part[bx] = red[0];
+        const int64_t sum = block_sum_tree(ovf, red);
+        if (threadIdx.x == 0) ovf_part[bx] = sum;
     }
     fft_stages<false>(buf, F, logN, tw);
     for (int idx = threadIdx.x; idx < (F << logN); idx += blockDim.x) {
@@ -589,13 +584,8 @@ __global__ void __launch_bounds__(kBlock) k_row_final(const JobDesc *__restrict_
         }
     }
     __shared__ double red[kBlock];
-    red[threadIdx.x] = e;
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) energy_part[blockIdx.x] = red[0];
+    const double sum = block_sum_tree(e, red);
+    if (threadIdx.x == 0) energy_part[blockIdx.x] = sum;
 }
 
 

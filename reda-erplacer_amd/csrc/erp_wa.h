@@ -24,15 +24,10 @@
 namespace erp {
 namespace {
 
+// the workgroup's HPWL partial (valid in thread 0)
 __device__ __forceinline__ double wa_block_sum(double v) {
     __shared__ double red[kBlock];
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    return red[0];
+    return block_sum_tree(v, red);
 }
 
 // one dimension of a two-pin net
