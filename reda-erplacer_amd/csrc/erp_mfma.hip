@@ -33,6 +33,7 @@
 #include <cmath>
 
 #include "erp_internal.h"
+#include "erp_kernels.h"
 #include "erp_timer.h"
 
 namespace erp {
@@ -432,14 +433,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
                     }
                 }
         if (a.energy_part && z == 2) {
-            double *red = reinterpret_cast<double *>(lds);
-            red[tid] = energy;
-            __syncthreads();
-            for (int s = 128; s > 0; s >>= 1) {
-                if (tid < s) red[tid] += red[tid + s];
-                __syncthreads();
-            }
-            if (tid == 0) a.energy_part[blockIdx.x] = red[0];
+            const double sum = block_sum_tree(energy, reinterpret_cast<double *>(lds));
+            if (tid == 0) a.energy_part[blockIdx.x] = sum;
         }
     }
 }
