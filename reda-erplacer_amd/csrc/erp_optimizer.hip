@@ -23,6 +23,29 @@ __device__ __forceinline__ double block_sum_f64(double v, double *red) {
     return r;
 }
 
+// Both sums of block_sum_f64 for two values at once (the same order, one
+// barrier); valid in thread 0, and red is not free again until a barrier.
+__device__ __forceinline__ void block_sum2_f64(double &a, double &b, double (*red)[kBlock / 64]) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a;
+        red[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    double ra = 0.0, rb = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        ra += red[0][w];
+        rb += red[1][w];
+    }
+    a = ra;
+    b = rb;
+}
+
 // eforce.rs:6-12: overlap of [x, x+s] with bin k, NOT clamped at 0
 __device__ __forceinline__ float tri_unclamped(float x, float s, int k, float bs) {
     const float bin_xl = 0.f + (float)k * bs;
@@ -227,7 +250,7 @@ __global__ void __launch_bounds__(kBlock) k_combine(const JobDesc *__restrict__ 
 // from gd.
 template <bool STORE_PARTS, bool READ_GWL, bool READ_GD>
 __global__ void __launch_bounds__(kBlock) k_cell_fused(const JobDesc *__restrict__ jd) {
-    __shared__ double red[kBlock / 64];
+    __shared__ double red[2][kBlock / 64];
     ERP_CELL_JOB();
     const DevState *__restrict__ st = c.st;
     double a = 0.0, b = 0.0;
@@ -242,8 +265,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_fused(const JobDesc *__restrict
         }
         bb_terms(c, i, g, a, b);
     }
-    a = block_sum_f64(a, red);
-    b = block_sum_f64(b, red);
+    block_sum2_f64(a, b, red);
     if (threadIdx.x == 0) {
         c.part_a[blk] = a;
         c.part_b[blk] = b;
