@@ -61,6 +61,10 @@ def kernel_bytes(name, d, grid, cells_pins):
         "pass1_row_dct": B * 32,
         # pass 1 and the WA net groups up to degree 256 in one launch
         "pass1_wa": B * 32 + (P2 + int(narrow.sum())) * 28 + int(narrow.size) * 8,
+        # MFMA form: read q_mov, q_fix (16), clear q_mov (8), write density (4) and
+        # the split row operand (four bf16 planes of half the row: 4)
+        "mfma_finalize": B * 32,
+        "mfma_finalize_wa": B * 32 + (P2 + int(narrow.sum())) * 28 + int(narrow.size) * 8,
         # read row transform (4), write three column-inverse maps (12)
         "pass2_col_spectral": B * 16,
         # read three maps (12) + density (4), write field x/y and potential (12)
@@ -80,14 +84,15 @@ def kernel_bytes(name, d, grid, cells_pins):
 
 def kernel_flops(name, grid, potential):
     """MFMA flops one launch of the split-BF16 spectral form executes: every
-    pass is two half-size products (even / odd frequencies, K = L/2) of three
+    product is two half-size products (even / odd frequencies, K = L/2) of three
     bf16 terms each, 3 m n^2 for a row pass and 3 m^2 n for a column pass, times
-    the transforms the launch carries (the inverses: 3, or 2 without the
-    potential map).  The useful FP32-equivalent work is a third of it."""
+    the products the launch carries (inverse columns: U and V, 2; inverse rows:
+    3, or 2 without the potential map).  The useful FP32-equivalent work is a
+    third of it."""
     m = n = grid
     nz = 3 if potential else 2
     return {"mfma_fwd_row": 3 * m * n * n, "mfma_fwd_col": 3 * m * m * n,
-            "mfma_inv_col": nz * 3 * m * m * n, "mfma_inv_row": nz * 3 * m * n * n}.get(name, 0)
+            "mfma_inv_col": 2 * 3 * m * m * n, "mfma_inv_row": nz * 3 * m * n * n}.get(name, 0)
 
 
 # timer name -> kernel function (without the k_ prefix) in the rocprofv3 summaries

@@ -167,44 +167,22 @@ int phase_scheduler(erp_ctx *ctx) {
     return rc;
 }
 
-// Workgroups of the WA launch on the graph's side branch (two per CU): its
-// loop over the plan leaves the spectral passes room on every CU (256: the
-// branch becomes the longer one; 1024: slightly slower; capping the
-// pin-gradient sum as well is slower).
-constexpr unsigned kSideBranchBlocks = 512;
-
-// One iteration (iteration >= 1) as a fused kernel sequence: what the graphs hold.
-// FFT form: the WA kernel runs inside pass 1's launch (k_pass1_wa), whose
-// two workgroups per CU leave room for it, on the one stream.  MFMA form with
-// fork (graph capture only): after the scatter, the WA kernel and the per-cell
-// sum of its pin gradients (k_pin2cell) run on a second branch beside the
-// spectral passes and join before the cell kernel, which then reads the summed
-// gradient.  WA only reads the positions; the results are bit-identical.
-int launch_fused_iteration(erp_ctx *ctx, bool fork = false) {
+// One iteration (iteration >= 1) as a fused kernel sequence: what the graphs
+// hold.  The WA kernel shares the first spectral launch on the one stream (no
+// fork and join): pass 1 in the FFT form (k_pass1_wa), the density finalize in
+// the MFMA form (k_finalize_wa); each leaves room on every CU for the WA
+// workgroups, and the cell kernel then sums the pin gradients itself.  WA only
+// reads the positions; the results are bit-identical to the phase sequence.
+int launch_fused_iteration(erp_ctx *ctx) {
     KernelTimer *t = ctx->timing ? ctx->timer : nullptr;
     int rc;
-    if (ctx->spectral_mode != ERP_SPECTRAL_MFMA_BF16X3) {
-        // FFT form (captured and eager alike): WA shares pass 1's launch on the
-        // one stream (no fork and join), and the cell kernel sums the pin
-        // gradients itself
-        if ((rc = run_scatter(ctx))) return rc;
+    if ((rc = run_scatter(ctx))) return rc;
+    if (ctx->spectral_mode == ERP_SPECTRAL_MFMA_BF16X3)
+        rc = launch_potential_mfma(ctx->job.pb, ctx->mfma, ctx->stream, t, djob(ctx), &ctx->dims);
+    else
         rc = launch_potential_fft(djob(ctx), ctx->dims, 1, ctx->stream, t, /*with_wa=*/true);
-        if (ctx->keep_q) ctx->q_dirty = true;
-        if (rc) return rc;
-        fork = false;
-    } else if (fork) {
-        if ((rc = run_scatter(ctx))) return rc;
-        ERP_HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
-        ERP_HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-        if ((rc = launch_wa(djob(ctx), ctx->dims, 1, ctx->side, nullptr, kSideBranchBlocks))) return rc;
-        if ((rc = launch_pin2cell(djob(ctx), ctx->dims, 1, ctx->side))) return rc;
-        ERP_HIP_TRY(hipEventRecord(ctx->ev_join, ctx->side));
-        if ((rc = run_spectral(ctx))) return rc;
-        ERP_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
-    } else {
-        if ((rc = launch_wa(djob(ctx), ctx->dims, 1, ctx->stream, t))) return rc;
-        if ((rc = run_potential(ctx))) return rc;
-    }
+    if (ctx->keep_q) ctx->q_dirty = true;
+    if (rc) return rc;
     // every form above ran this iteration's sort: the field gather follows its
     // order; the step's HPWL / overflow / energy and the scheduler's staged half
     // run beside it, the Nesterov update and the commit after the cell kernel
@@ -213,7 +191,7 @@ int launch_fused_iteration(erp_ctx *ctx, bool fork = false) {
     if (t) t->end();
     if (rc) return rc;
     if (t) t->begin("cell_fused");
-    rc = launch_cell_fused(djob(ctx), ctx->dims, 1, ctx->debug_parts, ctx->stream, fork, /*read_gd=*/true);
+    rc = launch_cell_fused(djob(ctx), ctx->dims, 1, ctx->debug_parts, ctx->stream, /*read_gd=*/true);
     if (t) t->end();
     if (rc) return rc;
     if (t) t->begin("scalar");
@@ -233,14 +211,9 @@ void drop_graphs(erp_ctx *ctx) {
 }
 
 int capture(erp_ctx *ctx, int iters, hipGraphExec_t *out) {
-    if (!ctx->side) {       // the WA branch's stream and events (contexts that never capture skip them)
-        ERP_HIP_TRY(hipStreamCreate(&ctx->side));
-        ERP_HIP_TRY(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
-        ERP_HIP_TRY(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
-    }
     ERP_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     int rc = ERP_OK;
-    for (int k = 0; k < iters && rc == ERP_OK; ++k) rc = launch_fused_iteration(ctx, true);
+    for (int k = 0; k < iters && rc == ERP_OK; ++k) rc = launch_fused_iteration(ctx);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -295,7 +268,7 @@ int erp::launch_batched_iteration(const JobDesc *d, const BatchDims &bd, int nj,
     if (!rc) rc = owner ? launch_scatter_owner(d, bd, nj, s) : launch_scatter(d, bd, nj, true, s);
     if (!rc) rc = launch_potential_fft(d, bd, nj, s, nullptr, /*with_wa=*/true);   // WA inside pass 1
     if (!rc) rc = launch_field_sorted(d, bd, nj, s, /*with_sched=*/true);
-    if (!rc) rc = launch_cell_fused(d, bd, nj, false, s, false, /*read_gd=*/true);
+    if (!rc) rc = launch_cell_fused(d, bd, nj, false, s, /*read_gd=*/true);
     if (!rc) rc = launch_scalar(d, nj, SC_NEST | SC_COMMIT, s);
     if (!rc) rc = launch_update(d, bd, nj, s);
     return rc;
@@ -334,7 +307,7 @@ size_t erp::estimate_ctx_bytes(const erp_design *d, const erp_config *cfg) {
     const size_t per_bin = 2 * 8 + 9 * 4;                             // q_mov/q_fix + float maps
     const size_t partials = 16 * (P / 64 + C / 128 + (size_t)cfg->num_bins_x) + 64 * 1024;
     size_t mfma = 0;
-    if (cfg->spectral_mode == ERP_SPECTRAL_MFMA_BF16X3)     // operator planes + 8 split maps
+    if (cfg->spectral_mode == ERP_SPECTRAL_MFMA_BF16X3)     // operator planes and split maps
         mfma = mfma_plan_estimate((int)cfg->num_bins_x, (int)cfg->num_bins_y) + 8 * B / 4096 * 4;
     return N * per_node + C * per_cell + P * per_pin + E * 4 + B * per_bin + partials + mfma +
            2 * 8 * (size_t)(cfg->num_bins_x + cfg->num_bins_y) + (1u << 20);
@@ -479,7 +452,7 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     ctx->device = cfg->device;
     DeviceGuard g(ctx->device);
     if (g.rc) { delete ctx; return g.rc; }
-    if (hipStreamCreate(&ctx->stream) != hipSuccess) {   // the side branch: at first capture
+    if (hipStreamCreate(&ctx->stream) != hipSuccess) {
         delete ctx;
         return fail(ERP_ERR_HIP, "hipStreamCreate failed");
     }
@@ -731,7 +704,7 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     ctx->dims.add(ctx->job);
     TRY(upload_job(ctx));
     if (ctx->spectral_mode == ERP_SPECTRAL_MFMA_BF16X3)
-        TRY(mfma_plan_create(&ctx->mfma, ctx->m, ctx->n, s));
+        TRY(mfma_plan_create(&ctx->mfma, ctx->m, ctx->n, ctx->job.pb.bw / ctx->job.pb.bh, s));
     ctx->timer = new KernelTimer(s);
     if (hipStreamSynchronize(s) != hipSuccess) { erp_ctx_destroy(ctx); return fail(ERP_ERR_HIP, "setup failed"); }
 #undef TRY
@@ -746,12 +719,8 @@ int erp_ctx_destroy(erp_ctx *ctx) {
     drop_graphs(ctx);
     if (ctx->mfma) mfma_plan_destroy(ctx->mfma);
     delete ctx->timer;
-    hipStream_t s = ctx->stream, side = ctx->side;
-    hipEvent_t e1 = ctx->ev_fork, e2 = ctx->ev_join;
+    hipStream_t s = ctx->stream;
     delete ctx;   // releases every DevBuf
-    if (e1) (void)hipEventDestroy(e1);
-    if (e2) (void)hipEventDestroy(e2);
-    if (side) (void)hipStreamDestroy(side);
     if (s) (void)hipStreamDestroy(s);
     return ERP_OK;
 }
@@ -1128,7 +1097,7 @@ int leaf_spectral(const float *x, float *out, int32_t m, int32_t n, int32_t mode
         MfmaPlan *plan = nullptr;
         int rc;
         if ((rc = upload(dx, x, B, L.s)) || (rc = dout.alloc(sizeof(float) * B)) ||
-            (rc = mfma_plan_create(&plan, m, n, L.s)))
+            (rc = mfma_plan_create(&plan, m, n, 1.0f, L.s)))
             return rc;
         // which: 0 dct2, 1 idct2, 2 idct_idxst, 3 idxst_idct
         rc = which == 0 ? launch_dct2_mfma(dx.as<float>(), dout.as<float>(), plan, L.s)

@@ -8,8 +8,6 @@ using namespace erp;
 struct erp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipStream_t side = nullptr;        // second branch of the captured iteration (WA)
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int64_t N = 0, C = 0, F = 0, P = 0, E = 0, n2 = 0, Pm = 0;
     int m = 0, n = 0;
     int64_t B = 0;

@@ -188,10 +188,16 @@ struct PotentialBufs {
 int potential_partials(int m);   // number of partials pass 1 / pass 3 write
 // Potential with MFMA dense transforms (erp_mfma.hip).
 struct MfmaPlan;
-int mfma_plan_create(MfmaPlan **plan, int m, int n, hipStream_t s);
+struct JobDesc;
+struct BatchDims;
+// wv_div: bw / bh, the aspect factor of the y frequency operator
+int mfma_plan_create(MfmaPlan **plan, int m, int n, float wv_div, hipStream_t s);
 void mfma_plan_destroy(MfmaPlan *plan);
+// jd / bd (the placement's device descriptor and its dimensions): run the WA
+// net groups inside the finalize launch
 int launch_potential_mfma(const PotentialBufs &b, MfmaPlan *plan, hipStream_t s,
-                          KernelTimer *t = nullptr);
+                          KernelTimer *t = nullptr, const JobDesc *jd = nullptr,
+                          const BatchDims *bd = nullptr);
 bool mfma_supported(int m, int n);             // at least 128 bins per axis
 int mfma_energy_partials(int m, int n);        // energy partials the MFMA pipeline writes
 int mfma_overflow_partials(int m);             // overflow partials the MFMA pipeline writes

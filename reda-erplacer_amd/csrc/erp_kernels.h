@@ -251,10 +251,9 @@ int launch_combine(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s)
 // beside it (the WA, overflow and energy partials must be complete).
 int launch_field_sorted(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
                         bool with_sched = false);
-// read_gwl: the per-cell WA gradient was summed by k_pin2cell already;
 // read_gd: the field force was gathered by launch_field_sorted already
 int launch_cell_fused(const JobDesc *d, const BatchDims &bd, int nj, bool store_parts,
-                      hipStream_t s, bool read_gwl = false, bool read_gd = false);
+                      hipStream_t s, bool read_gd = false);
 int launch_update(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
 // out[3j..3j+2] = hpwl, overflow, nonfinite of job j
 int launch_gather_state(const JobDesc *d, int nj, double *out, hipStream_t s);

@@ -13,6 +13,17 @@
 // (E + O at i, E - O at L-1-i).  Every product is then a pair of
 // (rows x L/2) x (L/2 x cols) GEMMs, K = L/2.
 //
+// The three inverses share one spectrum.  With b = a.inv (frequencies.rs:22-57:
+// Wx = wu inv / 2, Wy = wv inv / 2) the diagonal factors move into the
+// operators:
+//   xi_x = 4 [S_M diag(wu/2)] b [diag(eps) C_N^T] = 4 V [diag(eps) C_N^T]
+//   xi_y = 4 [C_M diag(eps)]  b [diag(wv/2) S_N^T] = 4 U [diag(wv/2) S_N^T]
+//   phi  = 4 [C_M diag(eps)]  b [diag(eps) C_N^T]  = 4 U [diag(eps) C_N^T]
+// so the inverse along x is two products of the one map b (U and V) and the
+// inverse along y three (xi_x from V, xi_y and phi from U): five products
+// where the three separate 2-D inverses take six, and the forward pass writes
+// one spectrum instead of three.  The scaled operators are made in FP64.
+//
 // Precision: operands are split into bf16 pairs (x = hi + lo, |lo| <= 2^-9 |x|)
 // and each product is hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16 with
 // FP32 accumulation (relative error ~2^-17 per term).  The operator matrices
@@ -22,19 +33,21 @@
 // hi / lo and laid out the way the next pass reads it -- four "planes" per map
 // ([product half e / o][hi / lo], K contiguous) -- so the GEMM kernel only
 // copies bf16 tiles into LDS:
-//   k_finalize : q -> rho (f32) and the row pass operand, rows folded along y
+//   k_finalize : q -> rho (f32) and the row pass operand, rows folded along y;
+//                in the iteration the WA net groups share its launch
 //   FWD_ROW    : -> the column pass operand, folded along x (each workgroup owns
 //                a band of rows and its mirror image, so it can fold)
-//   FWD_COL    : -> the three spectra (a.Wx eps, eps a.Wy, eps eps a.inv), split
-//                by the parity of the x frequency
-//   INV_COL    : -> the three column inverses, unfolded along x, split by the
-//                parity of the y frequency
+//   FWD_COL    : -> the spectrum b = a.inv, split by the parity of the x
+//                frequency
+//   INV_COL    : -> U and V, unfolded along x, split by the parity of the y
+//                frequency
 //   INV_ROW    : -> xi_x, xi_y (the interleaved field map), phi, energy
 #include <cmath>
 
 #include "erp_internal.h"
 #include "erp_kernels.h"
 #include "erp_timer.h"
+#include "erp_wa.h"
 
 namespace erp {
 
@@ -49,6 +62,9 @@ enum { FWD_ROW = 0, FWD_COL = 1, INV_COL = 2, INV_ROW = 3 };
 //   forward  [k'][i] = C[i][2k' + p]         (FE: p = 0, FO: p = 1)
 //   inverse  [i][k'] = C[i][2k' + p], S[...]  (CE, CO, SE, SO)
 enum { T_FE = 0, T_FO = 1, T_CE = 2, T_CO = 3, T_SE = 4, T_SO = 5, T_COUNT = 6 };
+// the scaled inverse operators of the potential pipeline along one axis, same
+// layout: [i][k'] = C[i][k] eps_k (k = 2k' + p) and S[i][k] w_k / 2
+enum { W_CE = 0, W_CO = 1, W_SE = 2, W_SO = 3, W_COUNT = 4 };
 
 constexpr int kTile = 64;               // output rows and columns of a workgroup (per product half)
 constexpr int kKb = 32;                 // K per stage: one 16x16x32 MFMA step
@@ -71,7 +87,9 @@ __host__ __device__ inline __bf16 *plane(const Split &s, int half, int hl) {
 }
 
 struct GemmArgs {
-    const __bf16 *op;         // operator planes of the transformed length
+    // operator of transform z: its even-half hi plane; the even lo, odd hi and
+    // odd lo planes follow at H * ldo apart (the layout of both tables)
+    const __bf16 *op[3];
     int H;                    // half length: the K of every product
     int ldo;                  // row pitch of the operator planes
     Split in[3];              // data operand per transform z
@@ -79,8 +97,8 @@ struct GemmArgs {
     float *dst[3];            // INV_ROW result per z; FWD_COL without spectra: natural a
     int dst_stride[3];        // INV_ROW: element stride (2 for a component of the field map)
     int m, n;
-    int sine_mask;            // bit z: transform z uses the sine operator
-    int spectral;             // FWD_COL: write the three spectra (else natural a into dst[0])
+    int sine_mask;            // bit z: transform z's operator is a sine (its unfold sign)
+    int spectral;             // FWD_COL: write the spectrum b = a.inv (else natural a into dst[0])
     float scale, two_pi_over_m, two_pi_over_n, wv_div;
     float *auv;               // FWD_COL: a in natural order (nullable)
     const float *rho;         // INV_ROW: density for the energy of z = 2
@@ -94,8 +112,8 @@ __device__ __forceinline__ T pick3(const T (&v)[3], int z) {
     return z == 0 ? v[0] : (z == 1 ? v[1] : v[2]);
 }
 
-__device__ __forceinline__ const __bf16 *op_plane(const GemmArgs &a, int t, int hl) {
-    return a.op + (size_t)(2 * t + hl) * a.H * a.ldo;
+__device__ __forceinline__ const __bf16 *op_plane(const GemmArgs &a, int z, int half, int hl) {
+    return pick3(a.op, z) + (size_t)(2 * half + hl) * a.H * a.ldo;
 }
 
 __device__ __forceinline__ void split8(const float *v, bf16x8 &hi, bf16x8 &lo) {
@@ -242,7 +260,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     const int hsel = BOTH ? 0 : (int)blockIdx.y;                 // single-half passes: which half
     const int m = a.m, n = a.n;
     const bool sine = (a.sine_mask >> z) & 1;
-    const int te = (MODE == FWD_ROW || MODE == FWD_COL) ? T_FE : (sine ? T_SE : T_CE);
 
     const int out_cols = (MODE == FWD_ROW || MODE == INV_ROW) ? n / 2 : n;
     const int nct = out_cols / kTile;
@@ -266,9 +283,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
             const __bf16 *pa, *pb;
             if (MODE == FWD_ROW || MODE == INV_ROW) {            // A = data rows, B = operator
                 pa = plane(d, half, hl) + (size_t)arow * d.ld;
-                pb = op_plane(a, te + half, hl) + (size_t)brow * a.ldo;
+                pb = op_plane(a, z, half, hl) + (size_t)brow * a.ldo;
             } else {                                             // A = operator, B = data columns
-                pa = op_plane(a, te + half, hl) + (size_t)arow * a.ldo;
+                pa = op_plane(a, z, half, hl) + (size_t)arow * a.ldo;
                 pb = plane(d, half, hl) + (size_t)brow * d.ld;
             }
             src[2 * h + hl] = pa + 8 * lc;
@@ -333,13 +350,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
         *reinterpret_cast<bf16x8 *>(plane(O, 1, 1) + at) = lo;
     } else if (MODE == FWD_COL) {
         // rows k' (x frequency u = 2k' + half), columns c (de-interleaved y
-        // frequency): the three spectra, split by the parity of u, K = k'
+        // frequency): the spectrum b = a.inv, split by the parity of u, K = k'
         to_lds(T, 0);
         __syncthreads();
         const int c = tid & 63, kb = 16 * (tid >> 6);
         const int cg = c0 + c;
         const int v = cg < n / 2 ? 2 * cg : 2 * (cg - n / 2) + 1;
-        float sp[3][16];
+        float sp[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             const int u = 2 * (r0 + kb + q) + hsel;
@@ -349,29 +366,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
                 a.dst[0][(size_t)u * n + v] = av;
                 continue;
             }
-            // frequencies.rs:22-57 in f32
+            // frequencies.rs:22-57 in f32 (the inverse operators carry wu / 2,
+            // wv / 2 and eps)
             const float wu = (float)u * a.two_pi_over_m;
             const float wv = (float)v * a.two_pi_over_n * a.wv_div;
             const float den = wu * wu + wv * wv;
             const float inv = (u == 0 && v == 0) ? 0.f : 1.f / den;
-            const float wx = wu * inv * 0.5f, wy = wv * inv * 0.5f;
-            const float eu = u ? 1.f : 0.5f, ev = v ? 1.f : 0.5f;
-            sp[0][q] = av * wx * ev;
-            sp[1][q] = eu * (av * wy);
-            sp[2][q] = eu * ev * (av * inv);
+            sp[q] = av * inv;
         }
         if (a.spectral) {
+            const Split &O = a.out[0];
+            const size_t at = (size_t)cg * O.ld + r0 + kb;
 #pragma unroll
-            for (int zz = 0; zz < 3; ++zz) {
-                const Split &O = a.out[zz];
-                const size_t at = (size_t)cg * O.ld + r0 + kb;
-#pragma unroll
-                for (int g = 0; g < 2; ++g) {
-                    bf16x8 hi, lo;
-                    split8(&sp[zz][8 * g], hi, lo);
-                    *reinterpret_cast<bf16x8 *>(plane(O, hsel, 0) + at + 8 * g) = hi;
-                    *reinterpret_cast<bf16x8 *>(plane(O, hsel, 1) + at + 8 * g) = lo;
-                }
+            for (int g = 0; g < 2; ++g) {
+                bf16x8 hi, lo;
+                split8(&sp[8 * g], hi, lo);
+                *reinterpret_cast<bf16x8 *>(plane(O, hsel, 0) + at + 8 * g) = hi;
+                *reinterpret_cast<bf16x8 *>(plane(O, hsel, 1) + at + 8 * g) = lo;
             }
         }
     } else if (MODE == INV_COL) {
@@ -439,53 +450,49 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     }
 }
 
-// Density from the fixed-point accumulators (epotential.rs:271-285): rho,
-// overflow partials per 4 rows, movable accumulators cleared; then the row
-// pass operand: rho folded along y (k with n-1-k), split.
-__global__ void __launch_bounds__(1024) k_finalize(PotentialBufs b, Split rows) {
-    extern __shared__ __attribute__((aligned(16))) float R[];   // 4 rows of rho
+// Density from the fixed-point accumulators (epotential.rs:271-285) of row r:
+// rho, the row's overflow partial, movable accumulators cleared; then the row
+// pass operand: rho folded along y (k with n-1-k), split.  One 256-thread
+// workgroup per row; R holds the row (n floats of LDS).
+__device__ __forceinline__ void finalize_row(const PotentialBufs &b, const Split &rows, int r, float *R) {
     const int n = b.n;
-    const size_t base = (size_t)blockIdx.x * 4 * n;
+    const size_t base = (size_t)r * n;
     const longlong2 *__restrict__ qf = reinterpret_cast<const longlong2 *>(b.q_fix + base);
     longlong2 *__restrict__ qm = reinterpret_cast<longlong2 *>(b.q_mov + base);
     float2 *__restrict__ rho = reinterpret_cast<float2 *>(b.rho + base);
-    const int pairs = 2 * n;                  // 4 rows as int64 pairs
     int64_t ovf = 0;
-#pragma unroll 2
-    for (int k = threadIdx.x; k < pairs; k += 1024) {
+    for (int k = threadIdx.x; k < n / 2; k += kBlock) {
         const longlong2 a = qm[k], f = qf[k];
         if (!b.keep_q) qm[k] = make_longlong2(0, 0);
         const int64_t t0 = a.x + f.x, t1 = a.y + f.y;
         ovf += (t0 > b.coef_q ? t0 - b.coef_q : 0) + (t1 > b.coef_q ? t1 - b.coef_q : 0);
-        const float2 r = make_float2((float)((double)t0 * b.inv_scale) * b.inv_bin_area,
+        const float2 v = make_float2((float)((double)t0 * b.inv_scale) * b.inv_bin_area,
                                      (float)((double)t1 * b.inv_scale) * b.inv_bin_area);
-        rho[k] = r;
-        R[2 * k] = r.x;
-        R[2 * k + 1] = r.y;
+        rho[k] = v;
+        R[2 * k] = v.x;
+        R[2 * k + 1] = v.y;
     }
-    __shared__ int64_t red[16];
+    __shared__ int64_t red[kBlock / 64];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) ovf += __shfl_xor(ovf, off);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ovf;
     __syncthreads();
     if (threadIdx.x == 0) {
         int64_t v = 0;
-        for (int w = 0; w < 16; ++w) v += red[w];
-        b.ovf_part[blockIdx.x] = v;
+        for (int w = 0; w < kBlock / 64; ++w) v += red[w];
+        b.ovf_part[r] = v;
     }
-    // fold: 4 rows x n/2 columns in runs of 8
-    const int runs = n / 16;                  // per row
-    for (int t = threadIdx.x; t < 4 * runs; t += 1024) {
-        const int rr = t / runs, k = 8 * (t % runs);
-        const float *Rr = R + (size_t)rr * n;
+    // fold: n/2 columns in runs of 8
+    for (int t = threadIdx.x; t < n / 16; t += kBlock) {
+        const int k = 8 * t;
         float e[8], o[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const float x0 = Rr[k + q], x1 = Rr[n - 1 - k - q];
+            const float x0 = R[k + q], x1 = R[n - 1 - k - q];
             e[q] = x0 + x1;
             o[q] = x0 - x1;
         }
-        const size_t at = (size_t)(blockIdx.x * 4 + rr) * rows.ld + k;
+        const size_t at = (size_t)r * rows.ld + k;
         bf16x8 hi, lo;
         split8(e, hi, lo);
         *reinterpret_cast<bf16x8 *>(plane(rows, 0, 0) + at) = hi;
@@ -494,6 +501,25 @@ __global__ void __launch_bounds__(1024) k_finalize(PotentialBufs b, Split rows) 
         *reinterpret_cast<bf16x8 *>(plane(rows, 1, 0) + at) = hi;
         *reinterpret_cast<bf16x8 *>(plane(rows, 1, 1) + at) = lo;
     }
+}
+
+__global__ void __launch_bounds__(kBlock) k_finalize(PotentialBufs b, Split rows) {
+    extern __shared__ __attribute__((aligned(16))) float R[];
+    finalize_row(b, rows, (int)blockIdx.x, R);
+}
+
+// The finalize and the WA net groups up to degree 256 in one launch:
+// workgroups [0, m) finalize a row each, the rest loop over the WA plan (the
+// same work and partial per plan workgroup as in its own launch).  The two are
+// independent (WA reads the positions, the finalize the accumulators), so they
+// overlap with no second stream and no events.
+__global__ void __launch_bounds__(kBlock) k_finalize_wa(const JobDesc *__restrict__ jd, Split rows, WaShare wa) {
+    extern __shared__ __attribute__((aligned(16))) float R[];
+    const PotentialBufs &b = jd->pb;
+    if ((int)blockIdx.x < b.m)
+        finalize_row(b, rows, (int)blockIdx.x, R);
+    else
+        wa_narrow_blocks(jd->wa, wa.plan, wa.first + (blockIdx.x - b.m), gridDim.x - b.m, wa.last);
 }
 
 // Leaf forward transform: a natural m x n map -> the row pass operand.
@@ -570,29 +596,59 @@ __global__ void k_make_planes(__bf16 *tab, int L) {
     tab[(size_t)(2 * t + 1) * H * ld + (size_t)r * ld + c] = lo;
 }
 
+// The scaled inverse operators of length L (W_CE, W_CO, W_SE, W_SO) in FP64,
+// split to bf16 pairs: [i][k'] = C[i][k] eps_k (eps_0 = 1/2, else 1) and
+// S[i][k] k w1 / 2, k = 2k' + p, where k w1 is the axis' frequency operator
+// w_k (frequencies.rs:22-57: 2 pi k / L, times bw / bh along y).
+__global__ void k_make_inv_planes(__bf16 *tab, int L, double w1) {
+    const int H = L / 2, ld = pitch_of(H);
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)W_COUNT * H * H) return;
+    const int t = (int)(idx / ((int64_t)H * H));
+    const int rem = (int)(idx % ((int64_t)H * H));
+    const int i = rem / H, k = 2 * (rem % H) + (t & 1);
+    const int64_t num = ((int64_t)(2 * i + 1) * k) % (4 * (int64_t)L);
+    double s, co;
+    sincospi((double)num / (double)(2 * L), &s, &co);
+    const double v = t < W_SE ? (k ? co : 0.5 * co) : s * (0.5 * w1 * (double)k);
+    const __bf16 hi = (__bf16)(float)v;
+    const __bf16 lo = (__bf16)(float)(v - (double)(float)hi);
+    tab[(size_t)(2 * t) * H * ld + (size_t)rem / H * ld + rem % H] = hi;
+    tab[(size_t)(2 * t + 1) * H * ld + (size_t)rem / H * ld + rem % H] = lo;
+}
+
 inline unsigned cdiv(int64_t a, int b) { return (unsigned)((a + b - 1) / b); }
 
 }  // namespace
 
 // Split maps between the passes: the row pass operand (m rows, K = n/2), the
-// column pass operand and the spectra (n rows, K = m/2), the column inverses
-// (m rows, K = n/2).
+// column pass operand and the spectrum (n rows, K = m/2), U and V (m rows,
+// K = n/2).
 struct MfmaPlan {
     int m = 0, n = 0;
     DevBuf tab_m, tab_n;          // operator planes (tab_n unused when m == n)
-    DevBuf row, col, x[3], y[3];
+    DevBuf inv_m, inv_n;          // scaled inverse operators along x and y (inv_n unused
+                                  // when m == n and bw == bh: the same operators)
+    DevBuf row, col, x, y[2];
     const __bf16 *tm() const { return tab_m.as<__bf16>(); }
     const __bf16 *tn() const { return m == n ? tab_m.as<__bf16>() : tab_n.as<__bf16>(); }
+    const __bf16 *im() const { return inv_m.as<__bf16>(); }
+    const __bf16 *in() const { return inv_n.p ? inv_n.as<__bf16>() : inv_m.as<__bf16>(); }
 };
 
 namespace {
 size_t table_bytes(int L) { return sizeof(__bf16) * 2 * T_COUNT * (size_t)(L / 2) * pitch_of(L / 2); }
+size_t inv_table_bytes(int L) { return sizeof(__bf16) * 2 * W_COUNT * (size_t)(L / 2) * pitch_of(L / 2); }
 size_t split_bytes(int rows, int k) { return sizeof(__bf16) * 4 * (size_t)rows * pitch_of(k); }
+// the hi plane of the even half of operator t of a table of length L
+const __bf16 *op_of(const __bf16 *tab, int L, int t) {
+    return tab + (size_t)(2 * t) * (L / 2) * pitch_of(L / 2);
+}
 }  // namespace
 
 size_t mfma_plan_estimate(int m, int n) {
-    return table_bytes(m) + (m == n ? 0 : table_bytes(n)) + 4 * split_bytes(m, n / 2) +
-           4 * split_bytes(n, m / 2);
+    return table_bytes(m) + (m == n ? 0 : table_bytes(n)) + inv_table_bytes(m) + inv_table_bytes(n) +
+           3 * split_bytes(m, n / 2) + 2 * split_bytes(n, m / 2);
 }
 
 bool mfma_supported(int m, int n) {
@@ -602,10 +658,10 @@ bool mfma_supported(int m, int n) {
 // the last inverse pass writes one energy partial per tile of the potential
 int mfma_energy_partials(int m, int n) { return (m / kTile) * ((n / 2) / kTile); }
 
-// k_finalize runs one workgroup per four rows and writes one overflow partial each.
-int mfma_overflow_partials(int m) { return m / 4; }
+// k_finalize runs one workgroup per row and writes one overflow partial each.
+int mfma_overflow_partials(int m) { return m; }
 
-int mfma_plan_create(MfmaPlan **out, int m, int n, hipStream_t s) {
+int mfma_plan_create(MfmaPlan **out, int m, int n, float wv_div, hipStream_t s) {
     if (!mfma_supported(m, n))
         return fail(ERP_ERR_UNSUPPORTED, "the MFMA spectral form needs at least 128 bins per axis");
     MfmaPlan *p = new MfmaPlan();
@@ -618,11 +674,19 @@ int mfma_plan_create(MfmaPlan **out, int m, int n, hipStream_t s) {
         hipLaunchKernelGGL(k_make_planes, dim3(cdiv(cnt, 256)), dim3(256), 0, s, b.as<__bf16>(), L);
         if (hipGetLastError() != hipSuccess) rc = fail(ERP_ERR_HIP, "k_make_planes launch failed");
     };
+    auto make_inv = [&](DevBuf &b, int L, double w1) {
+        const int64_t cnt = (int64_t)W_COUNT * (L / 2) * (L / 2);
+        if ((rc = b.alloc(inv_table_bytes(L)))) return;
+        hipLaunchKernelGGL(k_make_inv_planes, dim3(cdiv(cnt, 256)), dim3(256), 0, s, b.as<__bf16>(), L, w1);
+        if (hipGetLastError() != hipSuccess) rc = fail(ERP_ERR_HIP, "k_make_inv_planes launch failed");
+    };
     make(p->tab_m, m);
     if (!rc && n != m) make(p->tab_n, n);
-    for (DevBuf *b : {&p->row, &p->y[0], &p->y[1], &p->y[2]})
+    if (!rc) make_inv(p->inv_m, m, 2.0 * M_PI / (double)m);
+    if (!rc && (n != m || wv_div != 1.0f)) make_inv(p->inv_n, n, 2.0 * M_PI / (double)n * (double)wv_div);
+    for (DevBuf *b : {&p->row, &p->y[0], &p->y[1]})
         if (!rc) rc = b->alloc(split_bytes(m, n / 2));
-    for (DevBuf *b : {&p->col, &p->x[0], &p->x[1], &p->x[2]})
+    for (DevBuf *b : {&p->col, &p->x})
         if (!rc) rc = b->alloc(split_bytes(n, m / 2));
     if (rc) {
         delete p;
@@ -636,9 +700,8 @@ void mfma_plan_destroy(MfmaPlan *plan) { delete plan; }
 
 size_t mfma_plan_bytes(const MfmaPlan *p) {
     if (!p) return 0;
-    size_t t = p->tab_m.bytes + p->tab_n.bytes + p->row.bytes + p->col.bytes;
-    for (int z = 0; z < 3; ++z) t += p->x[z].bytes + p->y[z].bytes;
-    return t;
+    return p->tab_m.bytes + p->tab_n.bytes + p->inv_m.bytes + p->inv_n.bytes + p->row.bytes + p->col.bytes +
+           p->x.bytes + p->y[0].bytes + p->y[1].bytes;
 }
 
 namespace {
@@ -666,14 +729,14 @@ int launch_mode(const GemmArgs &a, int nz, hipStream_t s) {
     return ERP_OK;
 }
 
-// row-pass operand -> column-pass operand -> (spectra | natural a)
+// row-pass operand -> column-pass operand -> (spectrum | natural a)
 int forward(const MfmaPlan *p, Split rowop, Split colop, GemmArgs c, hipStream_t s, KernelTimer *t) {
     GemmArgs a{};
     a.m = p->m;
     a.n = p->n;
     a.in[0] = rowop;
     a.out[0] = colop;
-    a.op = p->tn();
+    a.op[0] = op_of(p->tn(), p->n, T_FE);
     a.H = p->n / 2;
     a.ldo = pitch_of(a.H);
     if (t) t->begin("mfma_fwd_row");
@@ -683,7 +746,7 @@ int forward(const MfmaPlan *p, Split rowop, Split colop, GemmArgs c, hipStream_t
     c.m = p->m;
     c.n = p->n;
     c.in[0] = colop;
-    c.op = p->tm();
+    c.op[0] = op_of(p->tm(), p->m, T_FE);
     c.H = p->m / 2;
     c.ldo = pitch_of(c.H);
     if (t) t->begin("mfma_fwd_col");
@@ -692,38 +755,55 @@ int forward(const MfmaPlan *p, Split rowop, Split colop, GemmArgs c, hipStream_t
     return rc;
 }
 
-// split spectra X[z] -> column inverses Y[z] -> row inverses out[z].
-int inverse(const MfmaPlan *p, const Split *X, const Split *Y, float *const *out,
-            const int *out_stride, int nz, int col_sine, int row_sine, const float *rho,
-            double *energy_part, hipStream_t s, KernelTimer *t) {
+// One inverse along x (INV_COL: split spectra -> split Y) then along y
+// (INV_ROW: split Y -> natural maps); per transform z its input, operator and
+// unfold sign (bit z of the sine masks).
+struct InverseSpec {
+    int ncol, nrow;                      // transforms of each pass
+    Split col_in[3], col_out[3];
+    const __bf16 *col_op[3];
+    int col_sine;
+    Split row_in[3];
+    const __bf16 *row_op[3];
+    int row_sine;
+    float *out[3];
+    int out_stride[3];
+    const float *rho;                    // energy of row transform 2 (nullable)
+    double *energy_part;
+};
+
+int inverse(const MfmaPlan *p, const InverseSpec &v, hipStream_t s, KernelTimer *t) {
     GemmArgs a{};
     a.m = p->m;
     a.n = p->n;
-    for (int z = 0; z < nz; ++z) { a.in[z] = X[z]; a.out[z] = Y[z]; }
-    a.op = p->tm();
+    for (int z = 0; z < v.ncol; ++z) {
+        a.in[z] = v.col_in[z];
+        a.out[z] = v.col_out[z];
+        a.op[z] = v.col_op[z];
+    }
     a.H = p->m / 2;
     a.ldo = pitch_of(a.H);
-    a.sine_mask = col_sine;
+    a.sine_mask = v.col_sine;
     if (t) t->begin("mfma_inv_col");
-    int rc = launch_mode<INV_COL>(a, nz, s);
+    int rc = launch_mode<INV_COL>(a, v.ncol, s);
     if (t) t->end();
     if (rc) return rc;
     GemmArgs b{};
     b.m = p->m;
     b.n = p->n;
-    for (int z = 0; z < nz; ++z) {
-        b.in[z] = Y[z];
-        b.dst[z] = out[z];
-        b.dst_stride[z] = out_stride ? out_stride[z] : 1;
+    for (int z = 0; z < v.nrow; ++z) {
+        b.in[z] = v.row_in[z];
+        b.op[z] = v.row_op[z];
+        b.dst[z] = v.out[z];
+        b.dst_stride[z] = v.out_stride[z];
     }
-    b.op = p->tn();
     b.H = p->n / 2;
     b.ldo = pitch_of(b.H);
-    b.sine_mask = row_sine;
-    b.rho = rho;
-    b.energy_part = energy_part;
+    b.sine_mask = v.row_sine;
+    b.rho = v.rho;
+    b.energy_part = v.energy_part;
     if (t) t->begin("mfma_inv_row");
-    rc = launch_mode<INV_ROW>(b, nz, s);
+    rc = launch_mode<INV_ROW>(b, v.nrow, s);
     if (t) t->end();
     return rc;
 }
@@ -731,35 +811,68 @@ int inverse(const MfmaPlan *p, const Split *X, const Split *Y, float *const *out
 }  // namespace
 
 // The potential pipeline of one placement (epotential.rs:225-374) in MFMA form.
-int launch_potential_mfma(const PotentialBufs &b, MfmaPlan *p, hipStream_t s, KernelTimer *t) {
+// With `jd` (the placement's device descriptor) the WA net groups run inside
+// the finalize launch (the wide buckets first, on their own).
+int launch_potential_mfma(const PotentialBufs &b, MfmaPlan *p, hipStream_t s, KernelTimer *t,
+                          const JobDesc *jd, const BatchDims *bd) {
     if (!p || p->m != b.m || p->n != b.n) return fail(ERP_ERR_INVALID, "MFMA plan does not match the grid");
     const int m = b.m, n = b.n;
     const Split rowop = split_of(p->row, m, n / 2);         // [half][hl][row][k < n/2]
     const Split colop = split_of(p->col, n, m / 2);         // [half][hl][column][k < m/2]
-    if (t) t->begin("mfma_finalize");
-    hipLaunchKernelGGL(k_finalize, dim3((unsigned)mfma_overflow_partials(m)), dim3(1024),
-                       sizeof(float) * 4 * (size_t)n, s, b, rowop);
-    if (t) t->end();
+    const size_t lds = sizeof(float) * (size_t)n;
+    int rc;
+    if (jd) {
+        if ((rc = launch_wa_wide(jd, *bd, 1, s))) return rc;
+        const WaPlan plan = wa_narrow_plan(*bd);
+        const WaShare wa{plan, 0u, plan.begin[plan.nseg]};
+        if (t) t->begin("mfma_finalize_wa");
+        hipLaunchKernelGGL(k_finalize_wa, dim3((unsigned)m + wa.last), dim3(kBlock), lds, s, jd, rowop, wa);
+        if (t) t->end();
+    } else {
+        if (t) t->begin("mfma_finalize");
+        hipLaunchKernelGGL(k_finalize, dim3((unsigned)m), dim3(kBlock), lds, s, b, rowop);
+        if (t) t->end();
+    }
     ERP_HIP_TRY(hipGetLastError());
     GemmArgs c{};
-    const Split X[3] = {split_of(p->x[0], n, m / 2), split_of(p->x[1], n, m / 2), split_of(p->x[2], n, m / 2)};
-    for (int z = 0; z < 3; ++z) c.out[z] = X[z];
+    const Split X = split_of(p->x, n, m / 2);
+    c.out[0] = X;
     c.spectral = 1;
     c.scale = (float)(4.0 / ((double)m * (double)n));
     c.two_pi_over_m = (float)(2.0 * M_PI) / (float)m;
     c.two_pi_over_n = (float)(2.0 * M_PI) / (float)n;
     c.wv_div = b.bw / b.bh;
     c.auv = b.auv;
-    int rc = forward(p, rowop, colop, c, s, t);
-    if (rc) return rc;
-    const Split Y[3] = {split_of(p->y[0], m, n / 2), split_of(p->y[1], m, n / 2), split_of(p->y[2], m, n / 2)};
+    if ((rc = forward(p, rowop, colop, c, s, t))) return rc;
+    // U = [C_M diag eps] b (z = 0), V = [S_M diag wu/2] b (z = 1); then
+    // xi_x = 4 V [diag eps C_N^T], xi_y = 4 U [diag wv/2 S_N^T], phi = 4 U [diag eps C_N^T]
+    const Split U = split_of(p->y[0], m, n / 2), V = split_of(p->y[1], m, n / 2);
+    const __bf16 *cm = op_of(p->im(), m, W_CE), *sm = op_of(p->im(), m, W_SE);
+    const __bf16 *cn = op_of(p->in(), n, W_CE), *sn = op_of(p->in(), n, W_SE);
     float *fxy = reinterpret_cast<float *>(b.fxy);
-    float *out[3] = {fxy, fxy + 1, b.pot};
-    const int stride[3] = {2, 2, 1};
-    // xi_x: sine along x, cosine along y; xi_y: cosine along x, sine along y; phi: cosines
-    // the potential (third inverse) and its energy only when requested
-    return inverse(p, X, Y, out, stride, b.potential ? 3 : 2, /*col_sine=*/1, /*row_sine=*/2, b.rho,
-                   b.potential ? b.energy_part : nullptr, s, t);
+    InverseSpec v{};
+    v.ncol = 2;
+    v.col_in[0] = v.col_in[1] = X;
+    v.col_out[0] = U;
+    v.col_out[1] = V;
+    v.col_op[0] = cm;
+    v.col_op[1] = sm;
+    v.col_sine = 2;
+    // the potential (third row transform) and its energy only when requested
+    v.nrow = b.potential ? 3 : 2;
+    v.row_in[0] = V;
+    v.row_in[1] = v.row_in[2] = U;
+    v.row_op[0] = v.row_op[2] = cn;
+    v.row_op[1] = sn;
+    v.row_sine = 2;
+    v.out[0] = fxy;
+    v.out[1] = fxy + 1;
+    v.out[2] = b.pot;
+    v.out_stride[0] = v.out_stride[1] = 2;
+    v.out_stride[2] = 1;
+    v.rho = b.rho;
+    v.energy_part = b.potential ? b.energy_part : nullptr;
+    return inverse(p, v, s, t);
 }
 
 // Leaf transforms (host-facing entry points run these on device buffers).
@@ -781,14 +894,23 @@ int launch_dct2_mfma(const float *x, float *out, MfmaPlan *p, hipStream_t s) {
 int launch_inverse_mfma(const float *x, float *out, int which, MfmaPlan *p, hipStream_t s) {
     const int m = p->m, n = p->n;
     const int eps_u = which != 2, eps_v = which != 1;
-    const Split X[1] = {split_of(p->x[0], n, m / 2)};
+    const Split X = split_of(p->x, n, m / 2);
     hipLaunchKernelGGL(k_prep_spectrum, dim3(cdiv((int64_t)n * 2 * (m / 16), 256)), dim3(256), 0, s, x,
-                       X[0], m, n, eps_u, eps_v);
+                       X, m, n, eps_u, eps_v);
     ERP_HIP_TRY(hipGetLastError());
-    const Split Y[1] = {split_of(p->y[0], m, n / 2)};
-    float *O[1] = {out};
-    return inverse(p, X, Y, O, nullptr, 1, which == 2 ? 1 : 0, which == 1 ? 1 : 0, nullptr, nullptr, s,
-                   nullptr);
+    const bool sine_x = which == 2, sine_y = which == 1;
+    InverseSpec v{};
+    v.ncol = v.nrow = 1;
+    v.col_in[0] = X;
+    v.col_out[0] = split_of(p->y[0], m, n / 2);
+    v.col_op[0] = op_of(p->tm(), m, sine_x ? T_SE : T_CE);
+    v.col_sine = sine_x;
+    v.row_in[0] = v.col_out[0];
+    v.row_op[0] = op_of(p->tn(), n, sine_y ? T_SE : T_CE);
+    v.row_sine = sine_y;
+    v.out[0] = out;
+    v.out_stride[0] = 1;
+    return inverse(p, v, s, nullptr);
 }
 
 }  // namespace erp

@@ -244,23 +244,21 @@ __global__ void __launch_bounds__(kBlock) k_combine(const JobDesc *__restrict__ 
     }
 }
 
-// READ_GWL: the WA gradient per cell was summed already (k_pin2cell, on the
-// graph branch that runs beside the spectral passes) and is read from gwl.
 // READ_GD: the field force was gathered already (k_field_sorted) and is read
 // from gd.
-template <bool STORE_PARTS, bool READ_GWL, bool READ_GD>
+template <bool STORE_PARTS, bool READ_GD>
 __global__ void __launch_bounds__(kBlock) k_cell_fused(const JobDesc *__restrict__ jd) {
     __shared__ double red[2][kBlock / 64];
     ERP_CELL_JOB();
     const DevState *__restrict__ st = c.st;
     double a = 0.0, b = 0.0;
     if (i < c.C) {
-        const float2 gwl = READ_GWL ? gld(c.gwl + i) : pin2cell(c, i);
+        const float2 gwl = pin2cell(c, i);
         const float2 gd = READ_GD ? gld(c.gd + i) : efield_cell(c, i, 1.f / c.bw, 1.f / c.bh);
         const float2 g = combine(c, i, gwl, gd, (float)st->density_weight);
         c.g[i] = g;
         if (STORE_PARTS) {
-            if (!READ_GWL) c.gwl[i] = gwl;
+            c.gwl[i] = gwl;
             if (!READ_GD) c.gd[i] = gd;
         }
         bb_terms(c, i, g, a, b);
@@ -650,21 +648,13 @@ int launch_field_sorted(const JobDesc *d, const BatchDims &bd, int nj, hipStream
     return ERP_OK;
 }
 int launch_cell_fused(const JobDesc *d, const BatchDims &bd, int nj, bool store_parts,
-                      hipStream_t s, bool read_gwl, bool read_gd) {
+                      hipStream_t s, bool read_gd) {
     if (read_gd) {
-        if (read_gwl) {
-            if (store_parts) ERP_LAUNCH_CELLS((k_cell_fused<true, true, true>), d);
-            ERP_LAUNCH_CELLS((k_cell_fused<false, true, true>), d);
-        }
-        if (store_parts) ERP_LAUNCH_CELLS((k_cell_fused<true, false, true>), d);
-        ERP_LAUNCH_CELLS((k_cell_fused<false, false, true>), d);
+        if (store_parts) ERP_LAUNCH_CELLS((k_cell_fused<true, true>), d);
+        ERP_LAUNCH_CELLS((k_cell_fused<false, true>), d);
     }
-    if (read_gwl) {
-        if (store_parts) ERP_LAUNCH_CELLS((k_cell_fused<true, true, false>), d);
-        ERP_LAUNCH_CELLS((k_cell_fused<false, true, false>), d);
-    }
-    if (store_parts) ERP_LAUNCH_CELLS((k_cell_fused<true, false, false>), d);
-    ERP_LAUNCH_CELLS((k_cell_fused<false, false, false>), d);
+    if (store_parts) ERP_LAUNCH_CELLS((k_cell_fused<true, false>), d);
+    ERP_LAUNCH_CELLS((k_cell_fused<false, false>), d);
 }
 int launch_update(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s) {
     ERP_LAUNCH_CELLS(k_update, d);

@@ -227,14 +227,9 @@ __global__ void __launch_bounds__(kBlock) k_pass1(const JobDesc *__restrict__ jd
                        (int)blockIdx.x);
 }
 
-// A range of the WA plan's workgroups run inside another launch (all of them
-// inside pass 1's: 4377 it/s at 1M cells against 4367 / 4345 / 4299 with 70 /
-// 50 / 30 % there and the rest inside pass 3's, whose 36 KB of LDS per
-// workgroup leave the WA workgroups little room).
-struct WaShare {
-    WaPlan plan;
-    unsigned first, last;   // plan workgroups [first, last)
-};
+// All of the WA plan's workgroups run inside pass 1's launch: 4377 it/s at 1M
+// cells against 4367 / 4345 / 4299 with 70 / 50 / 30 % there and the rest inside
+// pass 3's, whose 36 KB of LDS per workgroup leave the WA workgroups little room.
 
 // Pass 1 and part of the WA net groups up to degree 256 in one launch:
 // workgroups [0, npass) are pass-1 workgroups, the rest loop over their share

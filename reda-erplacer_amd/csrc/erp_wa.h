@@ -226,9 +226,14 @@ __device__ __forceinline__ void wa_large(const WaNets &w, int bucket, int64_t bl
 
 
 
-// A grid smaller than the plan loops over its workgroups (grid stride): the
-// graph branch that runs beside the spectral passes caps its grid so that it
-// does not crowd their workgroups out of the CUs.
+// A grid smaller than the plan loops over its workgroups (grid stride), as do
+// the WA workgroups that share a spectral pass's launch.
+
+// A range of the WA plan's workgroups run inside another launch.
+struct WaShare {
+    WaPlan plan;
+    unsigned first, last;   // plan workgroups [first, last)
+};
 
 // The workgroups first, first + stride, ... below `last` of a WA plan: segment
 // lookup, then the bucket's net group (the same work and partial as the

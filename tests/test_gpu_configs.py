@@ -118,8 +118,13 @@ def _check(r, mfma):
     assert r["hpwl"] < 1e-6
     assert r["wa_grad"] < 1e-5
     assert r["field_x"] < TOL_SPECTRAL and r["field_y"] < TOL_SPECTRAL
-    # the potential is discarded by Placement::step (placement.rs:185)
-    assert r["potential"] < (2 * TOL_SPECTRAL if mfma else TOL_SPECTRAL)
+    # The potential is discarded by Placement::step (placement.rs:185).  It
+    # weights the spectrum by 1/w^2, so its error is that of the lowest
+    # frequencies, which at a spread state are small beside the terms of their
+    # sums; in the split-bf16 products (FP32 accumulation over K = L/2) that
+    # error depends on the state reached: 5.8e-6 at the centre start, 2.1e-5
+    # after 800 iterations at 1024^2.
+    assert r["potential"] < (3 * TOL_SPECTRAL if mfma else TOL_SPECTRAL)
     # The gather reproduces the reference binary's f32 arithmetic: against the
     # f32 restatement on the GPU's own field the error is summation order only.
     # Near the centre start the terms of a cell's sum cancel to 1e-4..1e-5 of
