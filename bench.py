@@ -189,6 +189,23 @@ def serving_requests(n_requests, rate):
     return reqs
 
 
+def serving_profile(reqs, check_every, max_active, max_iterations):
+    """The saturated serving stream on an eager server (the batched kernels of
+    each round launched directly; results identical to the graph replay)."""
+    from erplacer import workloads as W
+    from erplacer.server import Server
+    stop = W.SERVING["stop_overflow"]
+    td = W.SERVING["target_density"]
+    t0 = time.perf_counter()
+    with Server(max_active=max_active, check_every=check_every, eager=True) as srv:
+        ids = [srv.submit(d, g, max_iterations, stop, target_density=td) for _, d, g in reqs]
+        res = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    conv = sum(1 for r in res if r["converged"])
+    print(f"[serving] eager stream: {len(res)} requests, {conv} converged, "
+          f"{time.perf_counter() - t0:.2f}s, stats {st}", file=sys.stderr, flush=True)
+
+
 def serving_bench(reqs, rates, check_every, max_active, max_iterations):
     """Serving form: the request stream (log-uniform 2k-150k cells, grids 64-512,
     stop at overflow < 0.1) placed (1) one after another alone, (2) all submitted at
@@ -393,6 +410,10 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device-to-device hipMemcpy after the loop, "
                          "the reference dispatch for rocprofv3 FETCH_SIZE / WRITE_SIZE scaling")
+    ap.add_argument("--serving-profile", action="store_true",
+                    help="only the serving stream, all requests at once, on a server that "
+                         "launches the batched kernels directly instead of replaying graphs "
+                         "(for rocprofv3 kernel traces of the batched kernels)")
     ap.add_argument("--serving-requests", type=int, default=512,
                     help="requests of the serving measurement (0 skips it)")
     ap.add_argument("--serving-rate", default="20,60",
@@ -410,6 +431,11 @@ def main():
     from erplacer import lib as L
     from erplacer import workloads as W
     from erplacer.placement import Placement
+
+    if args.serving_profile:
+        serving_profile(serving_requests(args.serving_requests, float(args.serving_rate.split(",")[0])),
+                        args.serving_check_every, args.serving_max_active, args.serving_max_iterations)
+        return
 
     t0 = time.perf_counter()
     d, grid, iters_cfg, td = W.make(args.workload)

@@ -201,6 +201,10 @@ typedef struct {
     int32_t max_active;         /* jobs resident at once */
     int32_t check_every;        /* iterations between convergence checks */
     int32_t device;
+    /* 0 (default): each batch's check_every iterations are one captured hipGraph;
+     * 1: the same batched kernels are launched directly each round (no capture;
+     * identical results, for kernel tracers) */
+    int32_t eager;
 } erp_server_config;
 
 typedef struct {

@@ -93,6 +93,7 @@ struct Group {
     DevBuf table;                   // JobDesc[slots]
     BatchDims dims;                 // launch geometry of the graph
     hipGraphExec_t graph = nullptr;
+    bool ready = false;             // eager mode: slots and table set, launched each round
     DevBuf state_dev;               // (hpwl, overflow, nonfinite) per member, written by the graph
     double *state_host = nullptr;   // pinned copy, read after each round
     int capacity = 0;
@@ -110,6 +111,7 @@ struct Group {
     void drop_graph() {
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
+        ready = false;
     }
 };
 
@@ -270,7 +272,7 @@ void join(Group &g, Job *j) {
     ++g.members;
     BatchDims one;
     one.add(j->ctx->job);
-    if (!g.recapture && g.graph && covers(g.dims, one)) {
+    if (!g.recapture && (g.graph || g.ready) && covers(g.dims, one)) {
         auto it = std::find(g.slots.begin(), g.slots.end(), nullptr);
         if (it != g.slots.end()) {
             *it = j;
@@ -345,6 +347,10 @@ int rebuild(erp_server *s, Group &g) {
         g.capacity = cap;
     }
     if ((rc = upload_table(s, g))) return rc;
+    if (s->cfg.eager) {             // launched directly by each round
+        g.ready = true;
+        return ERP_OK;
+    }
     ERP_HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < s->cfg.check_every && !rc; ++k)
         rc = launch_batched_iteration(g.table.as<JobDesc>(), g.dims, nj, s->stream);
@@ -376,9 +382,18 @@ int round(erp_server *s) {
     std::vector<hipStream_t> used;
     for (auto &kv : s->groups) {
         Group &g = kv.second;
-        if (!g.graph) continue;
+        if (!g.graph && !g.ready) continue;
         if (!g.stream) ERP_HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
-        ERP_HIP_TRY(hipGraphLaunch(g.graph, g.stream));
+        if (g.graph) {
+            ERP_HIP_TRY(hipGraphLaunch(g.graph, g.stream));
+        } else {                    // eager: the kernels the graph would hold, in its order
+            const int nj = (int)g.slots.size();
+            int rc = ERP_OK;
+            for (int k = 0; k < s->cfg.check_every && !rc; ++k)
+                rc = launch_batched_iteration(g.table.as<JobDesc>(), g.dims, nj, g.stream);
+            if (!rc) rc = launch_gather_state(g.table.as<JobDesc>(), nj, g.state_dev.as<double>(), g.stream);
+            if (rc) return rc;
+        }
         {
             std::lock_guard<std::mutex> lk(s->mu);
             ++s->batched_launches;
@@ -516,6 +531,7 @@ void erp_server_config_default(erp_server_config *cfg) {
     cfg->max_active = 64;
     cfg->check_every = 10;
     cfg->device = 0;
+    cfg->eager = 0;
 }
 
 int erp_server_create(const erp_server_config *cfg, erp_server **out) {

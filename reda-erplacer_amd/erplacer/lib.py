@@ -64,7 +64,7 @@ class StepInfo(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active", C.c_int32),
-                ("check_every", C.c_int32), ("device", C.c_int32)]
+                ("check_every", C.c_int32), ("device", C.c_int32), ("eager", C.c_int32)]
 
 
 class JobResult(C.Structure):

@@ -4,7 +4,8 @@
 inside the library admits submitted designs while job slots and the device
 memory budget allow, keeps their state resident, advances the jobs that share a
 bin grid together in one captured hipGraph of ``check_every`` batched
-iterations, and retires a job once its overflow drops below its threshold or it
+iterations (``eager=True``: the same kernels launched directly, for tracers),
+and retires a job once its overflow drops below its threshold or it
 reaches its iteration limit.  A job's result equals
 ``Placement.run_until(max_iterations, stop_overflow, check_every)`` on the same
 design alone, bit for bit.
@@ -32,7 +33,7 @@ def estimate_device_bytes(design, num_bins, target_density=float("nan"), **cfg) 
 
 class Server:
     def __init__(self, hbm_budget_bytes: int | None = None, max_active: int | None = None,
-                 check_every: int | None = None, device: int = 0):
+                 check_every: int | None = None, device: int = 0, eager: bool = False):
         self._lib = L.load()
         cfg = L.ServerConfig()
         self._lib.erp_server_config_default(C.byref(cfg))
@@ -43,6 +44,7 @@ class Server:
         if check_every is not None:
             cfg.check_every = int(check_every)
         cfg.device = device
+        cfg.eager = 1 if eager else 0
         self.config = cfg
         h = C.c_void_p()
         L.check(self._lib.erp_server_create(C.byref(cfg), C.byref(h)))

@@ -32,13 +32,15 @@ def _alone(d, grid, max_it, stop):
         return info, x, y, P.final_hpwl_dbu()
 
 
-@pytest.mark.parametrize("max_active", [3, 16])
-def test_alone_equals_mix(gpu_lib, max_active):
+@pytest.mark.parametrize("max_active,eager", [(3, False), (16, False), (16, True)])
+def test_alone_equals_mix(gpu_lib, max_active, eager):
+    """eager: the batched kernels launched directly each round instead of the
+    captured graph (the mode for kernel tracers); the same results."""
     from erplacer.server import Server
     jobs = _designs()
     max_it, stop = 237, 0.25        # not a multiple of CHECK: the last chunk is shorter
     alone = [_alone(d, g, max_it, stop) for d, g in jobs]
-    with Server(max_active=max_active, check_every=CHECK) as srv:
+    with Server(max_active=max_active, check_every=CHECK, eager=eager) as srv:
         ids = [srv.submit(d, g, max_it, stop, target_density=1.0) for d, g in jobs]
         res = {i: srv.wait(i) for i in ids}
         st = srv.stats()
