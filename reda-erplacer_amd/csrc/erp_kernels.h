@@ -226,12 +226,11 @@ struct BatchDims {
     void add(const JobDesc &j);         // widen to include job j
 };
 
-// max_blocks > 0 caps the grid (the kernels loop over the plan's workgroups)
 int launch_wa(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
-              KernelTimer *t = nullptr, unsigned max_blocks = 0);
+              KernelTimer *t = nullptr);
 // the degree buckets above 256 only (k_wa_wide); launch_wa runs both
 int launch_wa_wide(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
-                   KernelTimer *t = nullptr, unsigned max_blocks = 0);
+                   KernelTimer *t = nullptr);
 int launch_sort(const JobDesc *d, const BatchDims &bd, int nj, bool owner_items, hipStream_t s,
                 KernelTimer *t = nullptr);   // owner_items: also build the owner-scatter work list
 // copy_perm: the sorted cell ids (aux.w) are copied back into perm, and the
@@ -259,9 +258,8 @@ int launch_update(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s);
 int launch_gather_state(const JobDesc *d, int nj, double *out, hipStream_t s);
 int launch_scalar(const JobDesc *d, int nj, int ops, hipStream_t s);
 // with_wa: pass 1 shares its launch with the WA net groups up to degree 256
-// (the groups above run just before), their grid capped at wa_cap workgroups
-// when wa_cap > 0
+// (the groups above run just before)
 int launch_potential_fft(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
-                         KernelTimer *t = nullptr, bool with_wa = false, unsigned wa_cap = 0);
+                         KernelTimer *t = nullptr, bool with_wa = false);
 
 }  // namespace erp

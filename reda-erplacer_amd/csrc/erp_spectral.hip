@@ -697,7 +697,7 @@ int launch_spectral_leaf(const float *x, float *out, float *scratch, int m, int 
 }
 
 int launch_potential_fft(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
-                         KernelTimer *t, bool with_wa, unsigned wa_cap) {
+                         KernelTimer *t, bool with_wa) {
     const int m = bd.m, n = bd.n, F = kRowPairs, logN = ilog2(n), logM = ilog2(m), G = col_pairs(m);
     // pass 1 (with the WA nets: the wide buckets first, on their own)
     size_t l1 = row_lds(n, F);
@@ -708,7 +708,7 @@ int launch_potential_fft(const JobDesc *d, const BatchDims &bd, int nj, hipStrea
         const WaPlan plan = wa_narrow_plan(bd);
         const unsigned total = plan.begin[plan.nseg];
         const WaShare wa{plan, 0u, total};
-        const unsigned wa_grid = wa_cap ? std::min(total, wa_cap) : total;
+        const unsigned wa_grid = total;
         if ((rc = set_lds_limit((const void *)k_pass1_wa, l1))) return rc;
         if (t) t->begin("pass1_wa");
         hipLaunchKernelGGL(k_pass1_wa, dim3((unsigned)npass + wa_grid, nj), dim3(kBlock), l1, s, d, logN, F,

@@ -117,13 +117,12 @@ WaPlan wa_wide_plan(const BatchDims &bd) {
     return p;
 }
 
-int launch_wa_wide(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s, KernelTimer *t,
-                   unsigned max_blocks) {
+int launch_wa_wide(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s, KernelTimer *t) {
     const WaPlan wide = wa_wide_plan(bd);
     const unsigned total = wide.begin[wide.nseg];
     if (total) {
         if (t) t->begin("wa_wide");
-        const unsigned grid = max_blocks ? std::min(total, max_blocks) : total;
+        const unsigned grid = total;
         hipLaunchKernelGGL(k_wa_wide, dim3(grid, nj), dim3(kBlock), 0, s, d, wide);
         if (t) t->end();
         ERP_HIP_TRY(hipGetLastError());
@@ -131,18 +130,17 @@ int launch_wa_wide(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s,
     return ERP_OK;
 }
 
-int launch_wa(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s, KernelTimer *t,
-              unsigned max_blocks) {
+int launch_wa(const JobDesc *d, const BatchDims &bd, int nj, hipStream_t s, KernelTimer *t) {
     const WaPlan narrow = wa_narrow_plan(bd);
     const unsigned total = narrow.begin[narrow.nseg];
     if (total) {
         if (t) t->begin("wa_narrow");
-        const unsigned grid = max_blocks ? std::min(total, max_blocks) : total;
+        const unsigned grid = total;
         hipLaunchKernelGGL(k_wa_narrow, dim3(grid, nj), dim3(kBlock), 0, s, d, narrow);
         if (t) t->end();
         ERP_HIP_TRY(hipGetLastError());
     }
-    return launch_wa_wide(d, bd, nj, s, t, max_blocks);
+    return launch_wa_wide(d, bd, nj, s, t);
 }
 
 int launch_hpwl_dbu(const WaNets &w, int64_t num_nets, unsigned long long *out, hipStream_t s) {

@@ -130,17 +130,16 @@ def test_graph_loop_matches_phase_loop(gpu_lib, small):
 
 
 @pytest.mark.parametrize("mode", ["fft", "mfma"])
-def test_graph_loop_matches_phase_loop_capped_wa(gpu_lib, mode):
-    """At 350k cells the WA plan has more than twice as many workgroups as the
-    MFMA form's side-branch WA launch (capped at 512 workgroups, looping over the
-    plan); the FFT form runs the whole plan inside pass 1's launch.  Either loop
-    is bit-identical to the phase kernels (one WA launch, one workgroup per
-    block; the field gather in id order)."""
+def test_graph_loop_matches_phase_loop_shared_wa(gpu_lib, mode):
+    """At 350k cells the WA plan's workgroups (more than two per CU) run inside
+    the first spectral launch of the loop: pass 1 in the FFT form, the density
+    finalize in the MFMA form.  Either loop is bit-identical to the phase kernels
+    (one WA launch of its own; the field gather in id order)."""
     from erplacer import lib as L
     from erplacer import workloads as W
     from erplacer.placement import Placement
     d, g, _, td = W.make("large", scale=0.35)
-    assert math.ceil(d.n_2nets / 256) > 512        # the two-pin blocks alone exceed the cap
+    assert math.ceil(d.n_2nets / 256) > 512        # the two-pin blocks alone: over two per CU
     sm = L.SPECTRAL_FFT if mode == "fft" else L.SPECTRAL_MFMA_BF16X3
     A = Placement(d, g, target_density=td, spectral_mode=sm)
     B = Placement(d, g, target_density=td, spectral_mode=sm)
