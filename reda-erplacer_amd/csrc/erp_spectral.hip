@@ -603,6 +603,7 @@ constexpr int kRowPairs = 1;   // F: packed pairs of rows per workgroup (2 rows)
 inline int col_pairs(int m) { return 1; }   // one packed column pair per workgroup (more workgroups in flight)
 
 size_t row_lds(int n, int arrays) { return sizeof(float2) * ((size_t)n / 2 + (size_t)n + (size_t)arrays * n); }
+constexpr size_t kWaShareLds = 24 * 1024;   // largest pass-1 LDS that WA shares its launch with
 
 // Raise a kernel's dynamic-LDS limit once per (device, kernel) to the largest size
 // requested so far, so that captured launches never need the attribute call.
@@ -703,6 +704,14 @@ int launch_potential_fft(const JobDesc *d, const BatchDims &bd, int nj, hipStrea
     size_t l1 = row_lds(n, F);
     int rc;
     const int npass = m / (2 * F);
+    // A launch's dynamic LDS is reserved by each of its workgroups, the WA ones
+    // included: above kWaShareLds per workgroup (rows of 2048 bins: 40 KB) too
+    // few WA workgroups fit a CU, and WA runs in a launch of its own before
+    // pass 1 (2M cells at 2048^2: 147 us shared against 128 us apart).
+    if (with_wa && l1 > kWaShareLds) {
+        if ((rc = launch_wa(d, bd, nj, s, t))) return rc;
+        with_wa = false;
+    }
     if (with_wa) {
         if ((rc = launch_wa_wide(d, bd, nj, s))) return rc;
         const WaPlan plan = wa_narrow_plan(bd);
