@@ -66,10 +66,22 @@ enum { T_FE = 0, T_FO = 1, T_CE = 2, T_CO = 3, T_SE = 4, T_SO = 5, T_COUNT = 6 }
 // layout: [i][k'] = C[i][k] eps_k (k = 2k' + p) and S[i][k] w_k / 2
 enum { W_CE = 0, W_CO = 1, W_SE = 2, W_SO = 3, W_COUNT = 4 };
 
-constexpr int kTile = 64;               // output rows and columns of a workgroup (per product half)
 constexpr int kKb = 32;                 // K per stage: one 16x16x32 MFMA step
-constexpr int kPlane = kTile * kKb;     // bf16 per LDS operand tile (4 KB)
-constexpr int kTp = kTile + 1;          // padded row of the f32 epilogue tiles
+// Output tile of a workgroup (rows and columns per product half): TT = 64
+// (four waves of 32 x 32) or, on grids with many tiles, TT = 128 (eight waves
+// of 32 x 64): each workgroup reads its whole A and B panels from L2, so the
+// larger tile halves the L2 -> CU traffic per output, and the wider wave tile
+// reads fewer LDS fragments per MFMA.
+template <int TT>
+struct Tile {
+    static constexpr int WR = TT / 32;              // waves along the rows (32 rows each)
+    static constexpr int WC = 2;                    // waves along the columns
+    static constexpr int WJ = TT / (16 * WC);       // 16-column MFMA tiles per wave
+    static constexpr int THREADS = WR * WC * 64;
+    static constexpr int PLANE = TT * kKb;          // bf16 per LDS operand tile
+    static constexpr int TP = TT + 1;               // padded row of the f32 epilogue tiles
+};
+constexpr int kBigTileMin = 2048;       // both axes at least this long: TT = 128
 // Row pitch of every bf16 plane: K plus 64 (128 bytes), so that the 16 rows one
 // fragment load touches do not all fall into the same few L2 channels (a
 // power-of-two pitch maps them onto a quarter of the channels or fewer).
@@ -105,11 +117,35 @@ struct GemmArgs {
     double *energy_part;
 };
 
-// element z of a kernel-argument array, selected without dynamic indexing (which
-// would copy the arguments to scratch)
+// A kernel-argument value made opaque to the optimizer (kept in scalar
+// registers): a select between such values cannot be folded back into one load
+// at a computed offset, which would make the argument block addressable and
+// copy it to scratch memory in every lane.
+template <class T>
+__device__ __forceinline__ T opaque(T v) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "one or two scalar registers");
+    if constexpr (sizeof(T) == 8) {
+        unsigned long long u;
+        __builtin_memcpy(&u, &v, 8);
+        asm volatile("" : "+s"(u));
+        __builtin_memcpy(&v, &u, 8);
+    } else {
+        unsigned u;
+        __builtin_memcpy(&u, &v, 4);
+        asm volatile("" : "+s"(u));
+        __builtin_memcpy(&v, &u, 4);
+    }
+    return v;
+}
+__device__ __forceinline__ Split opaque(const Split &v) {
+    return Split{opaque(v.base), opaque(v.rows), opaque(v.ld)};
+}
+
+// element z (wave-uniform) of a kernel-argument array
 template <class T>
 __device__ __forceinline__ T pick3(const T (&v)[3], int z) {
-    return z == 0 ? v[0] : (z == 1 ? v[1] : v[2]);
+    const T v0 = opaque(v[0]), v1 = opaque(v[1]), v2 = opaque(v[2]);
+    return z == 0 ? v0 : (z == 1 ? v1 : v2);
 }
 
 __device__ __forceinline__ const __bf16 *op_plane(const GemmArgs &a, int z, int half, int hl) {
@@ -149,54 +185,63 @@ __device__ __forceinline__ int tile_of_block(int b, int nb, int nct) {
 
 
 
-template <int MODE>
+template <int MODE, int TT>
 struct GemmShape {
     static constexpr bool BOTH = MODE == INV_COL || MODE == INV_ROW;   // both product halves per workgroup
     static constexpr int NH = BOTH ? 2 : 1;
     static constexpr int NPL = 4 * NH;      // tiles per stage: (A, B) x half x (hi, lo)
-    static constexpr int DEPTH = BOTH ? 3 : 4;   // stages of global loads in flight
+    // a thread stages one 16-byte chunk of row THREADS/4 of each plane
+    static_assert(Tile<TT>::THREADS == 4 * TT, "one thread per row chunk of a tile");
+    static constexpr int NPLT = NPL;            // planes a thread stages
+    // stages of global loads in flight (TT = 128 with both halves: two, so the
+    // staging registers fit beside the 32 x 64 accumulators of a wave)
+    static constexpr int DEPTH = BOTH ? (TT == 64 ? 3 : 2) : 4;
+    static constexpr int KLOOP_BYTES = 2 * NPL * Tile<TT>::PLANE * 2;
+    static constexpr int EPI_BYTES = (MODE == INV_COL ? 2 : 1) * TT * Tile<TT>::TP * 4;
+    static constexpr int LDS_BYTES = KLOOP_BYTES > EPI_BYTES ? KLOOP_BYTES : EPI_BYTES;
 };
 
 // One stage's tiles: global -> registers, registers -> LDS (16 bytes per tile per thread).
-template <int NPL>
-__device__ __forceinline__ void load_set(u32x4 (&st)[NPL], const __bf16 *const (&src)[NPL], int koff) {
+template <int NPLT>
+__device__ __forceinline__ void load_set(u32x4 (&st)[NPLT], const __bf16 *const (&src)[NPLT], int koff) {
 #pragma unroll
-    for (int q = 0; q < NPL; ++q) st[q] = *reinterpret_cast<const u32x4 *>(src[q] + koff);
+    for (int q = 0; q < NPLT; ++q) st[q] = *reinterpret_cast<const u32x4 *>(src[q] + koff);
 }
-template <int NPL>
-__device__ __forceinline__ void store_set(const u32x4 (&st)[NPL], __bf16 *buf, int soff) {
+template <int NPLT, int PLANE>
+__device__ __forceinline__ void store_set(const u32x4 (&st)[NPLT], __bf16 *buf, int soff) {
 #pragma unroll
-    for (int q = 0; q < NPL; ++q) *reinterpret_cast<u32x4 *>(buf + q * kPlane + soff) = st[q];
+    for (int q = 0; q < NPLT; ++q) *reinterpret_cast<u32x4 *>(buf + q * PLANE + soff) = st[q];
 }
 
-// The products of one LDS stage: per half, 2 x 2 MFMA tiles of 16 x 16 per wave,
+// The products of one LDS stage: per half, 2 x WJ MFMA tiles of 16 x 16 per wave,
 // each hi*hi + hi*lo + lo*hi.
-template <int MODE>
-__device__ __forceinline__ void compute_stage(const __bf16 *buf, f32x4 (&acc)[GemmShape<MODE>::NH][2][2],
+template <int MODE, int TT>
+__device__ __forceinline__ void compute_stage(const __bf16 *buf,
+                                              f32x4 (&acc)[GemmShape<MODE, TT>::NH][2][Tile<TT>::WJ],
                                               int wr, int wc, int lane) {
-    constexpr int NH = GemmShape<MODE>::NH;
+    constexpr int NH = GemmShape<MODE, TT>::NH, PL = Tile<TT>::PLANE, WJ = Tile<TT>::WJ;
     const int fr = lane & 15, fc = lane >> 4;
 #pragma unroll
     for (int h = 0; h < NH; ++h) {
-        const __bf16 *A = buf + (2 * h) * kPlane;
-        const __bf16 *B = buf + (2 * NH + 2 * h) * kPlane;
-        bf16x8 ah[2], al[2], bh[2], bl[2];
+        const __bf16 *A = buf + (2 * h) * PL;
+        const __bf16 *B = buf + (2 * NH + 2 * h) * PL;
+        bf16x8 ah[2], al[2], bh[WJ], bl[WJ];
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             const int o = swz(wr * 32 + i * 16 + fr, fc);
             ah[i] = *reinterpret_cast<const bf16x8 *>(A + o);
-            al[i] = *reinterpret_cast<const bf16x8 *>(A + kPlane + o);
+            al[i] = *reinterpret_cast<const bf16x8 *>(A + PL + o);
         }
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int o = swz(wc * 32 + j * 16 + fr, fc);
+        for (int j = 0; j < WJ; ++j) {
+            const int o = swz(wc * 16 * WJ + j * 16 + fr, fc);
             bh[j] = *reinterpret_cast<const bf16x8 *>(B + o);
-            bl[j] = *reinterpret_cast<const bf16x8 *>(B + kPlane + o);
+            bl[j] = *reinterpret_cast<const bf16x8 *>(B + PL + o);
         }
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
+            for (int j = 0; j < WJ; ++j) {
                 acc[h][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[h][i][j], 0, 0, 0);
                 acc[h][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[h][i][j], 0, 0, 0);
                 acc[h][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[h][i][j], 0, 0, 0);
@@ -221,58 +266,79 @@ struct Sets {
 // Stage T of the K loop (a compile-time recursion): multiply the LDS stage T,
 // store stage T+1 from its register set into the other LDS stage, refill that
 // set with stage T+1+DEPTH.
-template <int MODE, int NK, int T>
-__device__ __forceinline__ void k_stages(Sets<GemmShape<MODE>::NPL> &st,
-                                         const __bf16 *const (&src)[GemmShape<MODE>::NPL], __bf16 *lds,
-                                         int soff, f32x4 (&acc)[GemmShape<MODE>::NH][2][2], int wr,
+template <int MODE, int NK, int TT, int T>
+__device__ __forceinline__ void k_stages(Sets<GemmShape<MODE, TT>::NPLT> &st,
+                                         const __bf16 *const (&src)[GemmShape<MODE, TT>::NPLT], __bf16 *lds,
+                                         int soff, f32x4 (&acc)[GemmShape<MODE, TT>::NH][2][Tile<TT>::WJ], int wr,
                                          int wc, int lane) {
     if constexpr (T < NK) {
-        constexpr int NPL = GemmShape<MODE>::NPL, DEPTH = GemmShape<MODE>::DEPTH;
-        compute_stage<MODE>(lds + (T & 1) * NPL * kPlane, acc, wr, wc, lane);
+        using G = GemmShape<MODE, TT>;
+        constexpr int NPL = G::NPL, NPLT = G::NPLT, DEPTH = G::DEPTH, PL = Tile<TT>::PLANE;
+        compute_stage<MODE, TT>(lds + (T & 1) * NPL * PL, acc, wr, wc, lane);
         if constexpr (T + 1 < NK) {
-            store_set<NPL>(st.template get<(T + 1) % DEPTH>(), lds + ((T + 1) & 1) * NPL * kPlane, soff);
+            store_set<NPLT, PL>(st.template get<(T + 1) % DEPTH>(), lds + ((T + 1) & 1) * NPL * PL, soff);
             if constexpr (T + 1 + DEPTH < NK)
-                load_set<NPL>(st.template get<(T + 1) % DEPTH>(), src, (T + 1 + DEPTH) * kKb);
+                load_set<NPLT>(st.template get<(T + 1) % DEPTH>(), src, (T + 1 + DEPTH) * kKb);
         }
         __syncthreads();
-        k_stages<MODE, NK, T + 1>(st, src, lds, soff, acc, wr, wc, lane);
+        k_stages<MODE, NK, TT, T + 1>(st, src, lds, soff, acc, wr, wc, lane);
     }
 }
 
-template <int MODE, int NK, int D>
-__device__ __forceinline__ void k_prologue(Sets<GemmShape<MODE>::NPL> &st,
-                                           const __bf16 *const (&src)[GemmShape<MODE>::NPL]) {
-    if constexpr (D < GemmShape<MODE>::DEPTH && D < NK) {
-        load_set<GemmShape<MODE>::NPL>(st.template get<D>(), src, D * kKb);
-        k_prologue<MODE, NK, D + 1>(st, src);
+template <int MODE, int NK, int TT, int D>
+__device__ __forceinline__ void k_prologue(Sets<GemmShape<MODE, TT>::NPLT> &st,
+                                           const __bf16 *const (&src)[GemmShape<MODE, TT>::NPLT]) {
+    if constexpr (D < GemmShape<MODE, TT>::DEPTH && D < NK) {
+        load_set<GemmShape<MODE, TT>::NPLT>(st.template get<D>(), src, D * kKb);
+        k_prologue<MODE, NK, TT, D + 1>(st, src);
     }
 }
 
-// NK: K stages (H / 32).
-template <int MODE, int NK>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_gemm(GemmArgs a) {
-    constexpr bool BOTH = GemmShape<MODE>::BOTH;
-    constexpr int NH = GemmShape<MODE>::NH, NPL = GemmShape<MODE>::NPL, DEPTH = GemmShape<MODE>::DEPTH;
-    __shared__ __attribute__((aligned(16))) __bf16 lds[2 * NPL * kPlane];
+// Workgroup sum of one double per thread in a fixed order (256 or 512 threads).
+template <int NT>
+__device__ __forceinline__ double wg_sum(double v, double *red) {
+    if constexpr (NT == kBlock) {
+        return block_sum_tree(v, red);
+    } else {
+        static_assert(NT == 2 * kBlock, "256 or 512 threads");
+        red[threadIdx.x] = v;
+        __syncthreads();
+        const double x = threadIdx.x < kBlock ? red[threadIdx.x] + red[threadIdx.x + kBlock] : 0.0;
+        __syncthreads();
+        return block_sum_tree(x, red);   // threads >= kBlock contribute 0
+    }
+}
+
+// NK: K stages (H / 32); TT: output tile (64 or 128).
+template <int MODE, int NK, int TT>
+__global__ void __launch_bounds__(Tile<TT>::THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_gemm(GemmArgs a) {
+    using G = GemmShape<MODE, TT>;
+    constexpr bool BOTH = G::BOTH;
+    constexpr int NH = G::NH, NPLT = G::NPLT, DEPTH = G::DEPTH;
+    constexpr int WC = Tile<TT>::WC, WJ = Tile<TT>::WJ, TP = Tile<TT>::TP, H2 = TT / 2;
+    constexpr int NT = Tile<TT>::THREADS;
+    __shared__ __attribute__((aligned(16))) unsigned char lds_raw[G::LDS_BYTES];
+    __bf16 *lds = reinterpret_cast<__bf16 *>(lds_raw);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = wave >> 1, wc = wave & 1;                     // 32 x 32 outputs per wave
+    const int wr = wave / WC, wc = wave % WC;                    // 32 x 16 WJ outputs per wave
     const int z = BOTH ? (int)blockIdx.y : 0;
     const int hsel = BOTH ? 0 : (int)blockIdx.y;                 // single-half passes: which half
     const int m = a.m, n = a.n;
     const bool sine = (a.sine_mask >> z) & 1;
 
     const int out_cols = (MODE == FWD_ROW || MODE == INV_ROW) ? n / 2 : n;
-    const int nct = out_cols / kTile;
+    const int nct = out_cols / TT;
     const int tile = tile_of_block(blockIdx.x, gridDim.x, nct);
     const int rt = tile / nct, ct = tile % nct;
-    const int r0 = rt * kTile, c0 = ct * kTile;
+    const int r0 = rt * TT, c0 = ct * TT;
 
-    // ---- operand rows this thread stages: row `lr` of each 64-row tile, chunk `lc`
+    // ---- operand rows this thread stages: row `lr` of each TT-row tile, chunk `lc`
     const int lr = tid >> 2, lc = tid & 3;
     // FWD_ROW: the workgroup's rows are a band and its mirror image
-    const int arow = MODE == FWD_ROW ? (lr < 32 ? rt * 32 + lr : m - rt * 32 - 64 + lr) : r0 + lr;
+    const int arow = MODE == FWD_ROW ? (lr < H2 ? rt * H2 + lr : m - rt * H2 - TT + lr) : r0 + lr;
     const int brow = c0 + lr;
-    const __bf16 *src[NPL];
+    const __bf16 *src[NPLT];
     const Split din = pick3(a.in, z);
 #pragma unroll
     for (int h = 0; h < NH; ++h) {
@@ -294,73 +360,76 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     }
     const int soff = swz(lr, lc);
 
-    f32x4 acc[NH][2][2];
+    f32x4 acc[NH][2][WJ];
 #pragma unroll
     for (int h = 0; h < NH; ++h)
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) acc[h][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int j = 0; j < WJ; ++j) acc[h][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     // K loop over a double-buffered LDS stage, one barrier per stage; the global
     // loads of stage t go to register set t % DEPTH and stay in flight for DEPTH
     // stages before they are stored to LDS
-    Sets<NPL> st;
-    k_prologue<MODE, NK, 0>(st, src);
-    store_set<NPL>(st.template get<0>(), lds, soff);
-    if constexpr (DEPTH < NK) load_set<NPL>(st.template get<0>(), src, DEPTH * kKb);
+    Sets<NPLT> st;
+    k_prologue<MODE, NK, TT, 0>(st, src);
+    store_set<NPLT, Tile<TT>::PLANE>(st.template get<0>(), lds, soff);
+    if constexpr (DEPTH < NK) load_set<NPLT>(st.template get<0>(), src, DEPTH * kKb);
     __syncthreads();
-    k_stages<MODE, NK, 0>(st, src, lds, soff, acc, wr, wc, lane);
+    k_stages<MODE, NK, TT, 0>(st, src, lds, soff, acc, wr, wc, lane);
 
     // ------------------------------------------------------------- epilogues
-    // lane holds C[row = wr*32 + i*16 + 4*(lane/16) + j][col = wc*32 + jj*16 + lane%16]
+    // lane holds C[row = wr*32 + i*16 + 4*(lane/16) + j][col = wc*16*WJ + jj*16 + lane%16]
     float *T = reinterpret_cast<float *>(lds);
     auto to_lds = [&](float *dstT, int h) {
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int jj = 0; jj < 2; ++jj)
+            for (int jj = 0; jj < WJ; ++jj)
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    dstT[(wr * 32 + i * 16 + 4 * (lane >> 4) + j) * kTp + wc * 32 + jj * 16 + (lane & 15)] =
+                    dstT[(wr * 32 + i * 16 + 4 * (lane >> 4) + j) * TP + wc * 16 * WJ + jj * 16 + (lane & 15)] =
                         acc[h][i][jj][j];
     };
 
     if (MODE == FWD_ROW) {
-        // fold the band with its mirror (rows l and 63 - l) for the column pass:
+        // fold the band with its mirror (rows l and TT-1-l) for the column pass:
         // its operand row = the row-transform column hsel * n/2 + c, K = the row
         to_lds(T, 0);
         __syncthreads();
-        const int c = tid & 63, lb = 8 * (tid >> 6);
-        float e[8], o[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const float v1 = T[(lb + q) * kTp + c], v2 = T[(63 - lb - q) * kTp + c];
-            e[q] = v1 + v2;
-            o[q] = v1 - v2;
-        }
+        const int c = tid % TT;
         const Split &O = a.out[0];
-        const size_t at = (size_t)(hsel * (n / 2) + c0 + c) * O.ld + rt * 32 + lb;
-        bf16x8 hi, lo;
-        split8(e, hi, lo);
-        *reinterpret_cast<bf16x8 *>(plane(O, 0, 0) + at) = hi;
-        *reinterpret_cast<bf16x8 *>(plane(O, 0, 1) + at) = lo;
-        split8(o, hi, lo);
-        *reinterpret_cast<bf16x8 *>(plane(O, 1, 0) + at) = hi;
-        *reinterpret_cast<bf16x8 *>(plane(O, 1, 1) + at) = lo;
+        for (int lb = 8 * (tid / TT); lb < H2; lb += 8 * (NT / TT)) {
+            float e[8], o[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const float v1 = T[(lb + q) * TP + c], v2 = T[(TT - 1 - lb - q) * TP + c];
+                e[q] = v1 + v2;
+                o[q] = v1 - v2;
+            }
+            const size_t at = (size_t)(hsel * (n / 2) + c0 + c) * O.ld + rt * H2 + lb;
+            bf16x8 hi, lo;
+            split8(e, hi, lo);
+            *reinterpret_cast<bf16x8 *>(plane(O, 0, 0) + at) = hi;
+            *reinterpret_cast<bf16x8 *>(plane(O, 0, 1) + at) = lo;
+            split8(o, hi, lo);
+            *reinterpret_cast<bf16x8 *>(plane(O, 1, 0) + at) = hi;
+            *reinterpret_cast<bf16x8 *>(plane(O, 1, 1) + at) = lo;
+        }
     } else if (MODE == FWD_COL) {
         // rows k' (x frequency u = 2k' + half), columns c (de-interleaved y
         // frequency): the spectrum b = a.inv, split by the parity of u, K = k'
         to_lds(T, 0);
         __syncthreads();
-        const int c = tid & 63, kb = 16 * (tid >> 6);
+        const int c = tid % TT;
         const int cg = c0 + c;
         const int v = cg < n / 2 ? 2 * cg : 2 * (cg - n / 2) + 1;
+        for (int kb = 16 * (tid / TT); kb < TT; kb += 16 * (NT / TT)) {
         float sp[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             const int u = 2 * (r0 + kb + q) + hsel;
-            const float av = a.scale * T[(kb + q) * kTp + c];
+            const float av = a.scale * T[(kb + q) * TP + c];
             if (a.auv) a.auv[(size_t)u * n + v] = av;
             if (!a.spectral) {
                 a.dst[0][(size_t)u * n + v] = av;
@@ -385,25 +454,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
                 *reinterpret_cast<bf16x8 *>(plane(O, hsel, 1) + at + 8 * g) = lo;
             }
         }
+        }
     } else if (MODE == INV_COL) {
         // unfold along x (rows i and m-1-i); the row pass reads rows, split by
         // the parity of the y frequency (column half), K = column within it
-        float *TE = T, *TO = T + kTile * kTp;
+        float *TE = T, *TO = T + TT * TP;
         to_lds(TE, 0);
         to_lds(TO, NH - 1);
         __syncthreads();
-        const int l = tid >> 2, cb = 16 * (tid & 3);
-        const int i = r0 + l, mi = m - 1 - i;
+        const int cb = 16 * (tid % (TT / 16));
         const int cg = c0 + cb;
         const int q = cg < n / 2 ? 0 : 1, k = cg - q * (n / 2);
+        const Split O = pick3(a.out, z);
+        for (int l = tid / (TT / 16); l < TT; l += NT / (TT / 16)) {
+        const int i = r0 + l, mi = m - 1 - i;
         float y0[16], y1[16];
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
-            const float e = TE[l * kTp + cb + t], o = TO[l * kTp + cb + t];
+            const float e = TE[l * TP + cb + t], o = TO[l * TP + cb + t];
             y0[t] = e + o;
             y1[t] = sine ? o - e : e - o;
         }
-        const Split O = pick3(a.out, z);
 #pragma unroll
         for (int g = 0; g < 2; ++g) {
             bf16x8 hi, lo;
@@ -415,6 +486,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
             *reinterpret_cast<bf16x8 *>(plane(O, q, 0) + a1) = hi;
             *reinterpret_cast<bf16x8 *>(plane(O, q, 1) + a1) = lo;
         }
+        }
     } else {
         // INV_ROW: unfold along y (columns i and n-1-i), x4; energy of phi
         double energy = 0.0;
@@ -423,11 +495,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int jj = 0; jj < 2; ++jj)
+            for (int jj = 0; jj < WJ; ++jj)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const int row = r0 + wr * 32 + i * 16 + 4 * (lane >> 4) + j;
-                    const int col = c0 + wc * 32 + jj * 16 + (lane & 15);
+                    const int col = c0 + wc * 16 * WJ + jj * 16 + (lane & 15);
                     const float pe = acc[0][i][jj][j], po = acc[NH - 1][i][jj][j];
                     const float x0 = 4.f * (pe + po);
                     const float x1 = 4.f * (sine ? po - pe : pe - po);
@@ -444,7 +516,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
                     }
                 }
         if (a.energy_part && z == 2) {
-            const double sum = block_sum_tree(energy, reinterpret_cast<double *>(lds));
+            const double sum = wg_sum<NT>(energy, reinterpret_cast<double *>(lds));
             if (tid == 0) a.energy_part[blockIdx.x] = sum;
         }
     }
@@ -652,11 +724,19 @@ size_t mfma_plan_estimate(int m, int n) {
 }
 
 bool mfma_supported(int m, int n) {
-    return m >= 2 * kTile && n >= 2 * kTile && m % (2 * kTile) == 0 && n % (2 * kTile) == 0;
+    return m >= 128 && n >= 128 && m % 128 == 0 && n % 128 == 0;
 }
 
+namespace {
+// the output tile of the GEMM passes of an m x n grid (powers of two)
+int mfma_tile(int m, int n) { return m >= kBigTileMin && n >= kBigTileMin ? 128 : 64; }
+}  // namespace
+
 // the last inverse pass writes one energy partial per tile of the potential
-int mfma_energy_partials(int m, int n) { return (m / kTile) * ((n / 2) / kTile); }
+int mfma_energy_partials(int m, int n) {
+    const int t = mfma_tile(m, n);
+    return (m / t) * ((n / 2) / t);
+}
 
 // k_finalize runs one workgroup per row and writes one overflow partial each.
 int mfma_overflow_partials(int m) { return m; }
@@ -715,14 +795,21 @@ template <int MODE>
 int launch_mode(const GemmArgs &a, int nz, hipStream_t s) {
     const int rows = (MODE == FWD_ROW || MODE == INV_ROW) ? a.m : a.m / 2;
     const int cols = (MODE == FWD_ROW || MODE == INV_ROW) ? a.n / 2 : a.n;
-    const dim3 grid((unsigned)((rows / kTile) * (cols / kTile)),
+    const int tt = mfma_tile(a.m, a.n);
+    const dim3 grid((unsigned)((rows / tt) * (cols / tt)),
                     (MODE == FWD_ROW || MODE == FWD_COL) ? 2u : (unsigned)nz);
+    if (tt == 128) {       // both lengths 2048 (K = 1024)
+        if (a.H / kKb != 32) return fail(ERP_ERR_UNSUPPORTED, "MFMA 128-bin tiles need lengths of 2048");
+        hipLaunchKernelGGL((k_gemm<MODE, 32, 128>), grid, dim3(Tile<128>::THREADS), 0, s, a);
+        ERP_HIP_TRY(hipGetLastError());
+        return ERP_OK;
+    }
     switch (a.H / kKb) {   // H = L/2 with L a power of two in [128, 2048]
-    case 2: hipLaunchKernelGGL((k_gemm<MODE, 2>), grid, dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL((k_gemm<MODE, 4>), grid, dim3(256), 0, s, a); break;
-    case 8: hipLaunchKernelGGL((k_gemm<MODE, 8>), grid, dim3(256), 0, s, a); break;
-    case 16: hipLaunchKernelGGL((k_gemm<MODE, 16>), grid, dim3(256), 0, s, a); break;
-    case 32: hipLaunchKernelGGL((k_gemm<MODE, 32>), grid, dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_gemm<MODE, 2, 64>), grid, dim3(Tile<64>::THREADS), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_gemm<MODE, 4, 64>), grid, dim3(Tile<64>::THREADS), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_gemm<MODE, 8, 64>), grid, dim3(Tile<64>::THREADS), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((k_gemm<MODE, 16, 64>), grid, dim3(Tile<64>::THREADS), 0, s, a); break;
+    case 32: hipLaunchKernelGGL((k_gemm<MODE, 32, 64>), grid, dim3(Tile<64>::THREADS), 0, s, a); break;
     default: return fail(ERP_ERR_UNSUPPORTED, "MFMA transform length must be a power of two in [128, 2048]");
     }
     ERP_HIP_TRY(hipGetLastError());
