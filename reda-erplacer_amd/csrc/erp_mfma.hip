@@ -106,7 +106,7 @@ struct GemmArgs {
     int ldo;                  // row pitch of the operator planes
     Split in[3];              // data operand per transform z
     Split out[3];             // FWD_ROW / FWD_COL / INV_COL: split result per z
-    float *dst[3];            // INV_ROW result per z; FWD_COL without spectra: natural a
+    float *dst[3];            // INV_ROW result per z; FWD_COL with spectral == 0: natural a
     int dst_stride[3];        // INV_ROW: element stride (2 for a component of the field map)
     int m, n;
     int sine_mask;            // bit z: transform z's operator is a sine (its unfold sign)
@@ -842,8 +842,8 @@ int forward(const MfmaPlan *p, Split rowop, Split colop, GemmArgs c, hipStream_t
     return rc;
 }
 
-// One inverse along x (INV_COL: split spectra -> split Y) then along y
-// (INV_ROW: split Y -> natural maps); per transform z its input, operator and
+// One inverse along x (INV_COL: the split spectrum -> split U and V, one
+// product of the spectrum each) then along y (INV_ROW: U, V -> natural maps); per transform z its input, operator and
 // unfold sign (bit z of the sine masks).
 struct InverseSpec {
     int ncol, nrow;                      // transforms of each pass
