@@ -623,7 +623,7 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     TRY(ctx->sratio.alloc(sizeof(float) * (size_t)std::max<int64_t>(C, 1)));
     TRY(ctx->sll.alloc(sizeof(float2) * (size_t)std::max<int64_t>(C, 1)));
     TRY(launch_iota(ctx->perm.as<int32_t>(), C, s));
-    TRY(ctx->hist.alloc(sizeof(int32_t) * (size_t)kSortTiles));
+    TRY(ctx->hist.alloc(sizeof(int32_t) * (size_t)(kSortTiles + 1)));   // + the ticket
     TRY(ctx->cursor.alloc(sizeof(int32_t) * (size_t)kSortTiles));
     TRY(ctx->tstart.alloc(sizeof(int32_t) * (size_t)(kSortTiles + 1)));
     TRY(ctx->items.alloc(sizeof(int32_t) * (size_t)(kSortTiles + 1)));
@@ -640,6 +640,7 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
         sb.bh = ctx->bh;
         sb.keys = ctx->keys.as<int32_t>();
         sb.hist = ctx->hist.as<int32_t>();
+        sb.ticket = ctx->hist.as<int32_t>() + kSortTiles;
         sb.cursor = ctx->cursor.as<int32_t>();
         sb.perm = ctx->perm.as<int32_t>();
 
@@ -675,7 +676,7 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
         ctx->owner_auto = ctx->owner_scatter;
         ctx->reach = sb.reach;
     }
-    for (DevBuf *b : {&ctx->hpwl_part, &ctx->ovf_part, &ctx->energy_part, &ctx->part_a, &ctx->part_b, &ctx->hist})
+    for (DevBuf *b : {&ctx->hpwl_part, &ctx->ovf_part, &ctx->energy_part, &ctx->part_a, &ctx->part_b, &ctx->hist, &ctx->cursor})
         if (hipMemsetAsync(b->p, 0, b->bytes, s) != hipSuccess) { erp_ctx_destroy(ctx); return fail(ERP_ERR_HIP, "hipMemsetAsync failed"); }
     TRY(launch_setup_cells(ctx->d_w.as<float>(), ctx->d_h.as<float>(), ctx->d_npins_all.as<float>(),
                            N, C, ctx->bw, ctx->bh, ctx->t, die_w, die_h, ctx->csize.as<float2>(),

@@ -164,11 +164,15 @@ __global__ void __launch_bounds__(kScatterThreads) k_scatter(const JobDesc *__re
 // so that consecutive tiles are close) of their clamped lower-left corner.  The
 // input order is the previous iteration's sorted order, so each workgroup sees
 // cells of only a few tiles and the global histogram / reservation atomics stay
-// few.  Pass 1: keys + histogram.  Pass 2: exclusive scan into cursors (and a
-// cleared histogram for the next sort).  Pass 3: per-workgroup ranks in LDS, one
-// reservation atomic per tile and workgroup, and the sorted scatter records.
-// Order inside a tile depends on the schedule; the fixed-point scatter is order
-// independent, so the density map is not.
+// few.  Pass 1: keys + histogram.  Pass 2: per-workgroup ranks in LDS, one
+// reservation atomic per tile and workgroup, and the sorted scatter records;
+// every workgroup scans the tile counts itself (4096 at most, from L2) and
+// reserves relative to its tile's start, and the last workgroup of a job to
+// finish clears the counts and the cursors for the next sort.  The owner
+// scatter also needs each tile's start and its work items: for it a separate
+// scan (k_sort_scan) runs between the two passes and the cursors start at the
+// tile starts.  Order inside a tile depends on the schedule; the fixed-point
+// scatter is order independent, so the density map is not.
 __device__ __forceinline__ unsigned spread_bits(unsigned v) {   // 0b abc -> 0b a0b0c
     v &= 0xffu;
     v = (v | (v << 4)) & 0x0f0fu;
@@ -251,8 +255,8 @@ __global__ void __launch_bounds__(kScatterThreads) k_sort_keys(const JobDesc *__
         if (lh[t]) atomicAdd(&hist[t], lh[t]);
 }
 
-// Exclusive scan of the tile counts into the placement cursors and the tile
-// starts; then, for the owner scatter, the number of workgroups each tile needs
+// Owner scatter only: exclusive scan of the tile counts into the placement
+// cursors and the tile starts; then the number of workgroups each tile needs
 // (its own cells and those of the `reach` tiles to its left and below, in chunks
 // of kOwnerVisits; 0 outside the grid) and their exclusive scan.
 __device__ __forceinline__ int block_exclusive_scan(int sum, int *wsum, int &total) {
@@ -275,8 +279,7 @@ __device__ __forceinline__ int block_exclusive_scan(int sum, int *wsum, int &tot
     return before + inc - sum;
 }
 
-__global__ void __launch_bounds__(1024) k_sort_scan(const JobDesc *__restrict__ jd, int ntiles, int shift,
-                                                   int owner_items) {
+__global__ void __launch_bounds__(1024) k_sort_scan(const JobDesc *__restrict__ jd, int ntiles, int shift) {
     if (gridDim.x > 1 && !jd[blockIdx.x].active) return;   // an empty slot of a batch
     __shared__ int wsum[16];
     __shared__ int cnt[kSortTiles];
@@ -302,8 +305,7 @@ __global__ void __launch_bounds__(1024) k_sort_scan(const JobDesc *__restrict__ 
         }
     }
     if (threadIdx.x == 0) sb.tstart[ntiles] = total;
-    if (!owner_items) return;                        // uniform
-    // owner-scatter work items (cnt is complete: the scan above synchronised)
+    // work items (cnt is complete: the scan above synchronised)
     const int T = 1 << shift, R = sb.reach;
     const int tiles_x = (sb.m + T - 1) >> shift, tiles_y = (sb.n + T - 1) >> shift;
     int nch[4] = {0, 0, 0, 0};
@@ -331,9 +333,19 @@ __global__ void __launch_bounds__(1024) k_sort_scan(const JobDesc *__restrict__ 
     if (threadIdx.x == 0) sb.items[ntiles] = total;
 }
 
+// OWN: the workgroup scans the tile counts itself and the cursors count from 0
+// (else k_sort_scan has set each cursor to its tile's start).  Either way the
+// last workgroup of the job leaves the counts and the cursors at 0: every other
+// workgroup has used its loads of the counts and the results of its cursor
+// atomics before it takes its ticket.
+template <bool OWN>
 __global__ void __launch_bounds__(kScatterThreads) k_sort_place(const JobDesc *__restrict__ jd, int ntiles) {
     if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
+    constexpr int kPer = kSortTiles / kScatterThreads;      // tile counts per thread
     __shared__ int lh[kSortTiles];
+    __shared__ int pre[OWN ? kSortTiles : 1];
+    __shared__ int wsum[kScatterThreads / 64];
+    __shared__ int last;
     const SortBufs &sb = jd[blockIdx.y].sb;
     const int64_t num_cells = sb.num_cells;
     const int64_t nblk = (num_cells + kScatterCells - 1) / kScatterCells;
@@ -343,10 +355,17 @@ __global__ void __launch_bounds__(kScatterThreads) k_sort_place(const JobDesc *_
     const float2 *__restrict__ ll = sb.ll;
     const float4 *__restrict__ sstat = sb.sstat;
     const float *__restrict__ sratio = sb.sratio;
+    int32_t *__restrict__ hist = sb.hist;
     int32_t *__restrict__ cursor = sb.cursor;
 
     float4 *__restrict__ rec = sb.rec;
     float4 *__restrict__ aux = sb.aux;
+    const int t0 = threadIdx.x * kPer;
+    int hv[kPer];
+    if (OWN) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) hv[k] = t0 + k < ntiles ? hist[t0 + k] : 0;
+    }
     for (int t = threadIdx.x; t < ntiles; t += kScatterThreads) lh[t] = 0;
     __syncthreads();
     const int64_t base = xcd_block(blockIdx.x, nblk) * kScatterCells;
@@ -365,6 +384,18 @@ __global__ void __launch_bounds__(kScatterThreads) k_sort_place(const JobDesc *_
         cll[j] = valid ? ll[i] : make_float2(0.f, 0.f);
         cst[j] = valid ? sstat[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         crat[j] = valid ? sratio[i] : 0.f;
+    }
+    if (OWN) {   // exclusive prefix of the tile counts: each tile's first sorted index
+        int sum = 0;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) sum += hv[k];
+        int total;
+        int run = block_exclusive_scan(sum, wsum, total);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            if (t0 + k < ntiles) pre[t0 + k] = run;
+            run += hv[k];
+        }
     }
 #pragma unroll
     for (int j = 0; j < kScatterPer; ++j) {
@@ -385,7 +416,7 @@ __global__ void __launch_bounds__(kScatterThreads) k_sort_place(const JobDesc *_
     }
     __syncthreads();
     for (int t = threadIdx.x; t < ntiles; t += kScatterThreads)
-        if (lh[t]) lh[t] = atomicAdd(&cursor[t], lh[t]);   // count -> reserved start
+        if (lh[t]) lh[t] = (OWN ? pre[t] : 0) + atomicAdd(&cursor[t], lh[t]);   // count -> start
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kScatterPer; ++j) {
@@ -394,6 +425,15 @@ __global__ void __launch_bounds__(kScatterThreads) k_sort_place(const JobDesc *_
             rec[dst] = make_float4(cll[j].x, cll[j].y, cst[j].x, cst[j].y);
             aux[dst] = make_float4(cst[j].z, cst[j].w, crat[j], __int_as_float(cperm[j]));
         }
+    }
+    if (threadIdx.x == 0) last = atomicAdd(sb.ticket, 1) == (int)(nblk - 1);
+    __syncthreads();
+    if (last) {   // uniform: every workgroup of the job has passed its reservations
+        for (int t = threadIdx.x; t < ntiles; t += kScatterThreads) {
+            hist[t] = 0;
+            cursor[t] = 0;
+        }
+        if (threadIdx.x == 0) atomicExch(sb.ticket, 0);
     }
 }
 
@@ -605,12 +645,17 @@ int launch_sort(const JobDesc *d, const BatchDims &bd, int nj, bool owner_items,
     hipLaunchKernelGGL(k_sort_keys, dim3(nb, nj), dim3(kScatterThreads), 0, s, d, g.shift, g.ntiles);
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
-    if (t) t->begin("sort_scan");
-    hipLaunchKernelGGL(k_sort_scan, dim3(nj), dim3(1024), 0, s, d, g.ntiles, g.shift, owner_items ? 1 : 0);
-    if (t) t->end();
-    ERP_HIP_TRY(hipGetLastError());
+    if (owner_items) {
+        if (t) t->begin("sort_scan");
+        hipLaunchKernelGGL(k_sort_scan, dim3(nj), dim3(1024), 0, s, d, g.ntiles, g.shift);
+        if (t) t->end();
+        ERP_HIP_TRY(hipGetLastError());
+    }
     if (t) t->begin("sort_place");
-    hipLaunchKernelGGL(k_sort_place, dim3(nb, nj), dim3(kScatterThreads), 0, s, d, g.ntiles);
+    if (owner_items)
+        hipLaunchKernelGGL(k_sort_place<false>, dim3(nb, nj), dim3(kScatterThreads), 0, s, d, g.ntiles);
+    else
+        hipLaunchKernelGGL(k_sort_place<true>, dim3(nb, nj), dim3(kScatterThreads), 0, s, d, g.ntiles);
     if (t) t->end();
     ERP_HIP_TRY(hipGetLastError());
     return ERP_OK;

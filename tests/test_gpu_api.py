@@ -12,8 +12,9 @@ from conftest import rel_l2
 pytestmark = pytest.mark.gpu
 
 # the per-kernel timers of the FFT form's iteration (WA runs inside pass 1's launch,
-# the field gather in the sorted order before the cell kernel)
-FUSED_KERNELS = {"sort_keys", "sort_scan", "sort_place", "scatter", "pass1_wa",
+# the field gather in the sorted order before the cell kernel; the tile scan runs
+# inside the sort's placement pass)
+FUSED_KERNELS = {"sort_keys", "sort_place", "scatter", "pass1_wa",
                  "pass2_col_spectral", "pass3_row_final", "field_sorted", "cell_fused",
                  "scalar", "update"}
 
