@@ -623,7 +623,7 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
     TRY(ctx->sratio.alloc(sizeof(float) * (size_t)std::max<int64_t>(C, 1)));
     TRY(ctx->sll.alloc(sizeof(float2) * (size_t)std::max<int64_t>(C, 1)));
     TRY(launch_iota(ctx->perm.as<int32_t>(), C, s));
-    TRY(ctx->hist.alloc(sizeof(int32_t) * (size_t)(kSortTiles + 1)));   // + the ticket
+    TRY(ctx->hist.alloc(sizeof(int32_t) * (size_t)kSortTiles));
     TRY(ctx->cursor.alloc(sizeof(int32_t) * (size_t)kSortTiles));
     TRY(ctx->tstart.alloc(sizeof(int32_t) * (size_t)(kSortTiles + 1)));
     TRY(ctx->items.alloc(sizeof(int32_t) * (size_t)(kSortTiles + 1)));
@@ -640,7 +640,6 @@ int erp_ctx_create(const erp_design *d, const erp_config *cfg, erp_ctx **out) {
         sb.bh = ctx->bh;
         sb.keys = ctx->keys.as<int32_t>();
         sb.hist = ctx->hist.as<int32_t>();
-        sb.ticket = ctx->hist.as<int32_t>() + kSortTiles;
         sb.cursor = ctx->cursor.as<int32_t>();
         sb.perm = ctx->perm.as<int32_t>();
 

@@ -64,6 +64,8 @@ __global__ void __launch_bounds__(kScatterThreads) k_scatter(const JobDesc *__re
     __shared__ unsigned long long tile[kScatterTile];
     __shared__ int wbox[4][kScatterThreads / 64];
     const SortBufs &sb = jd[blockIdx.y].sb;
+    if (copy_perm && blockIdx.x == 0)   // after the sort: its tile counts start at 0 next time
+        for (int t = threadIdx.x; t < kSortTiles; t += kScatterThreads) sb.hist[t] = 0;
     const int64_t num_cells = sb.num_cells;
     const int64_t base = (int64_t)blockIdx.x * kScatterCells;
     if (base >= num_cells) return;                      // this job needs fewer workgroups
@@ -167,8 +169,8 @@ __global__ void __launch_bounds__(kScatterThreads) k_scatter(const JobDesc *__re
 // few.  Pass 1: keys + histogram.  Pass 2: per-workgroup ranks in LDS, one
 // reservation atomic per tile and workgroup, and the sorted scatter records;
 // every workgroup scans the tile counts itself (4096 at most, from L2) and
-// reserves relative to its tile's start, and the last workgroup of a job to
-// finish clears the counts and the cursors for the next sort.  The owner
+// reserves relative to its tile's start; pass 1 clears the cursors and the
+// scatter clears the counts for the next sort.  The owner
 // scatter also needs each tile's start and its work items: for it a separate
 // scan (k_sort_scan) runs between the two passes and the cursors start at the
 // tile starts.  Order inside a tile depends on the schedule; the fixed-point
@@ -215,7 +217,10 @@ __global__ void __launch_bounds__(kScatterThreads) k_sort_keys(const JobDesc *__
     int32_t *__restrict__ hist = sb.hist;
     const int m = sb.m, n = sb.n;
     const float inv_bw = 1.f / sb.bw, inv_bh = 1.f / sb.bh;
-    for (int t = threadIdx.x; t < ntiles; t += kScatterThreads) lh[t] = 0;
+    for (int t = threadIdx.x; t < ntiles; t += kScatterThreads) {
+        lh[t] = 0;
+        if (blockIdx.x == 0) sb.cursor[t] = 0;   // the placement pass's cursors count from 0
+    }
     __syncthreads();
     const int64_t base = xcd_block(blockIdx.x, nblk) * kScatterCells;
     // all loads of the thread's cells first (they are then in flight together)
@@ -334,10 +339,8 @@ __global__ void __launch_bounds__(1024) k_sort_scan(const JobDesc *__restrict__ 
 }
 
 // OWN: the workgroup scans the tile counts itself and the cursors count from 0
-// (else k_sort_scan has set each cursor to its tile's start).  Either way the
-// last workgroup of the job leaves the counts and the cursors at 0: every other
-// workgroup has used its loads of the counts and the results of its cursor
-// atomics before it takes its ticket.
+// (k_sort_keys cleared them; the scatter that follows clears the counts), else
+// k_sort_scan has set each cursor to its tile's start and cleared the counts.
 template <bool OWN>
 __global__ void __launch_bounds__(kScatterThreads) k_sort_place(const JobDesc *__restrict__ jd, int ntiles) {
     if (gridDim.y > 1 && !jd[blockIdx.y].active) return;   // an empty slot of a batch
@@ -345,7 +348,6 @@ __global__ void __launch_bounds__(kScatterThreads) k_sort_place(const JobDesc *_
     __shared__ int lh[kSortTiles];
     __shared__ int pre[OWN ? kSortTiles : 1];
     __shared__ int wsum[kScatterThreads / 64];
-    __shared__ int last;
     const SortBufs &sb = jd[blockIdx.y].sb;
     const int64_t num_cells = sb.num_cells;
     const int64_t nblk = (num_cells + kScatterCells - 1) / kScatterCells;
@@ -355,7 +357,7 @@ __global__ void __launch_bounds__(kScatterThreads) k_sort_place(const JobDesc *_
     const float2 *__restrict__ ll = sb.ll;
     const float4 *__restrict__ sstat = sb.sstat;
     const float *__restrict__ sratio = sb.sratio;
-    int32_t *__restrict__ hist = sb.hist;
+    const int32_t *__restrict__ hist = sb.hist;
     int32_t *__restrict__ cursor = sb.cursor;
 
     float4 *__restrict__ rec = sb.rec;
@@ -425,15 +427,6 @@ __global__ void __launch_bounds__(kScatterThreads) k_sort_place(const JobDesc *_
             rec[dst] = make_float4(cll[j].x, cll[j].y, cst[j].x, cst[j].y);
             aux[dst] = make_float4(cst[j].z, cst[j].w, crat[j], __int_as_float(cperm[j]));
         }
-    }
-    if (threadIdx.x == 0) last = atomicAdd(sb.ticket, 1) == (int)(nblk - 1);
-    __syncthreads();
-    if (last) {   // uniform: every workgroup of the job has passed its reservations
-        for (int t = threadIdx.x; t < ntiles; t += kScatterThreads) {
-            hist[t] = 0;
-            cursor[t] = 0;
-        }
-        if (threadIdx.x == 0) atomicExch(sb.ticket, 0);
     }
 }
 

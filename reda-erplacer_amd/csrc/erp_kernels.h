@@ -79,9 +79,6 @@ struct SortBufs {
     float bw, bh;
     int32_t *perm;       // [C] current order (input of the sort)
     int32_t *keys, *hist, *cursor;   // keys [C], hist/cursor [kSortTiles]
-    // workgroups of the sort's placement pass that have finished (the last one
-    // clears hist, cursor and this counter)
-    int32_t *ticket;
     float4 *rec;         // [C] sorted scatter records {clamped lower-left, clamped size}
     // [C] sorted side records {clamp offset x, y, ratio, cell id (int bits)}: one
     // 16-byte write per cell in the sort (the cell id is copied back into perm
